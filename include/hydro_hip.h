@@ -1,0 +1,170 @@
+/*
+ * hydro_hip.h -- C ABI of the MI355X (gfx950) first-order finite-volume Euler solver.
+ *
+ * Plain C: pointers, sizes and an opaque stream (a hipStream_t passed as void*, NULL =
+ * the default stream).  Every device pointer is HIP device memory laid out as in the
+ * reference (hydro2d/main.cxx:118-128, hydro3d/main.cxx:122-133):
+ *   2D: cell n = i*(NB_Y+2) + j;          rhoE[2n] = rho, rhoE[2n+1] = E,
+ *       uv[2n] = rho*u, uv[2n+1] = rho*v
+ *   3D: cell n = i*(NB_Y+2)*(NB_Z+2) + j*(NB_Z+2) + k;  rhoE as in 2D,
+ *       uvw[3n+{0,1,2}] = rho*u, rho*v, rho*w
+ * with one ghost layer on every side; NB_* are interior sizes.
+ *
+ * Errors: functions returning int give 0 on success and a negative code on failure;
+ * the launcher-style functions returning double give the elapsed microseconds, or a
+ * negative value on failure.  hydro_last_error() describes the last failure of the
+ * calling thread.
+ */
+#ifndef HYDRO_HIP_H_
+#define HYDRO_HIP_H_
+
+#include <stddef.h>
+
+#define HYDRO_HIP_ABI_VERSION 1
+
+/* Boundary modes (the reference hard-wires X periodic, Y and Z reflective:
+ * hydro2d/main.cxx:349-378 and hydro2d/kernel.cxx:213-251). */
+#define HYDRO_BC_PERIODIC 0
+#define HYDRO_BC_REFLECTIVE 1
+
+#define HYDRO_ERR_ARG (-1)
+#define HYDRO_ERR_HIP (-2)
+#define HYDRO_ERR_NONFINITE (-3)
+#define HYDRO_ERR_NOMEM (-4)
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------
+ * Drop-in launchers.
+ * ------------------------------------------------------------------------------- */
+
+/* Replaces `launcher` of hydro2d/kernel.hpp:63-68 (defined at hydro2d/kernel.cxx:331-367).
+ * Same parameter order and meaning; sycl::queue becomes a hipStream_t and the
+ * `const DATATYPE&` ratios are passed by value.  Reads d_rhoE/d_uv including the X
+ * ghost planes the caller filled, writes only the interior cells of d_*_next and
+ * *Dt_next = C_min / max signal speed of the new state, with reflective walls at
+ * j = 1 and j = NB_Y.  Synchronous like the reference (it waits for the kernel) and
+ * returns the kernel's elapsed microseconds (HIP events). */
+double launcher(const double* d_rhoE, const double* d_uv, double* d_rhoE_next, double* d_uv_next, double* Dt_next,
+                double C_min, double gamma, double gamma_minus_one, double divgamma, double K, size_t NB_X,
+                size_t NB_Y, double DtDx, double DtDy, void* stream);
+
+/* Replaces `launcher` of the 3D build, hydro3d/kernel.hpp:64-69 (hydro3d/kernel.cxx:
+ * 419-455), renamed so that one library carries both dimensions.  Reflective walls at
+ * j = 1, NB_Y and k = 1, NB_Z. */
+double launcher3d(const double* d_rhoE, const double* d_uvw, double* d_rhoE_next, double* d_uvw_next,
+                  double* Dt_next, double C_min, double gamma, double gamma_minus_one, double divgamma, double K,
+                  size_t NB_X, size_t NB_Y, size_t NB_Z, double DtDx, double DtDy, double DtDz, void* stream);
+
+/* The same two launchers for the USE_FLOAT build (hydro2d/kernel.hpp:9-23,
+ * hydro2d/Makefile:7-9): every state value and constant is float. */
+double launcher_f32(const float* d_rhoE, const float* d_uv, float* d_rhoE_next, float* d_uv_next, float* Dt_next,
+                    float C_min, float gamma, float gamma_minus_one, float divgamma, float K, size_t NB_X,
+                    size_t NB_Y, float DtDx, float DtDy, void* stream);
+double launcher3d_f32(const float* d_rhoE, const float* d_uvw, float* d_rhoE_next, float* d_uvw_next, float* Dt_next,
+                      float C_min, float gamma, float gamma_minus_one, float divgamma, float K, size_t NB_X,
+                      size_t NB_Y, size_t NB_Z, float DtDx, float DtDy, float DtDz, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Asynchronous forms with boundary modes (no wait, no timing).
+ * ------------------------------------------------------------------------------- */
+
+/* One step as `launcher`, enqueued on `stream`.  bc_y selects reflective walls
+ * (HYDRO_BC_REFLECTIVE, the reference) or reading the Y ghost cells
+ * (HYDRO_BC_PERIODIC, filled by hydro_ghost_fill2d). */
+int hydro_step2d(const double* d_rhoE, const double* d_uv, double* d_rhoE_next, double* d_uv_next, double* Dt_next,
+                 double C_min, double gamma, double gamma_minus_one, double divgamma, double K, size_t NB_X,
+                 size_t NB_Y, double DtDx, double DtDy, int bc_y, void* stream);
+int hydro_step3d(const double* d_rhoE, const double* d_uvw, double* d_rhoE_next, double* d_uvw_next,
+                 double* Dt_next, double C_min, double gamma, double gamma_minus_one, double divgamma, double K,
+                 size_t NB_X, size_t NB_Y, size_t NB_Z, double DtDx, double DtDy, double DtDz, int bc_y, int bc_z,
+                 void* stream);
+
+/* Ghost fill on the device.  Replaces update_ghost_cells_single_device
+ * (hydro2d/main.cxx:349-378, hydro3d/main.cxx:374-405) together with the host->device
+ * ghost-plane copies (hydro2d/main.cxx:719-729) and, for a domain resident on one GPU,
+ * the MPI halo exchange (hydro2d/main.cxx:381-424).
+ *   bc_x PERIODIC:   plane 0 <- plane NB_X, plane NB_X+1 <- plane 1 (all j, k).
+ *   bc_x REFLECTIVE: plane 0 <- mirror(plane 1), plane NB_X+1 <- mirror(plane NB_X),
+ *                    rho*u negated.
+ *   bc_y / bc_z PERIODIC: Y (Z) ghost lines <- the opposite interior line, in every
+ *                    plane, after the X planes.  REFLECTIVE: Y (Z) ghosts untouched
+ *                    (the walls are applied inside the step kernel). */
+int hydro_ghost_fill2d(double* d_rhoE, double* d_uv, size_t NB_X, size_t NB_Y, int bc_x, int bc_y, void* stream);
+int hydro_ghost_fill3d(double* d_rhoE, double* d_uvw, size_t NB_X, size_t NB_Y, size_t NB_Z, int bc_x, int bc_y,
+                       int bc_z, void* stream);
+int hydro_ghost_fill2d_f32(float* d_rhoE, float* d_uv, size_t NB_X, size_t NB_Y, int bc_x, int bc_y, void* stream);
+int hydro_ghost_fill3d_f32(float* d_rhoE, float* d_uvw, size_t NB_X, size_t NB_Y, size_t NB_Z, int bc_x, int bc_y,
+                           int bc_z, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Device-resident step loop: the host loop of hydro2d/main.cxx:686-815 with Dt,
+ * t and the iteration count kept on the device, so that steps enqueue back to back
+ * with no host round trip.  Results are identical to calling hydro_ghost_fill* and
+ * launcher* per step and advancing t on the host.
+ * ------------------------------------------------------------------------------- */
+typedef struct hydro_run hydro_run;
+
+/* dim = 2 or 3; nb[dim] interior sizes; D[dim] the cell spacings DX, DY(, DZ) as the
+ * reference computes them (ONE / (NB + 1), hydro2d/main.cxx:482-483); C_min = C *
+ * min(D); dt0 the initial Dt; the loop runs while t < max_t and it < max_it.  The
+ * per-iteration (t, Dt) pairs of up to max_it steps are recorded on the device.
+ * precision: 64 = FP64 (reference), 32 = the USE_FLOAT build.  Returns NULL on error. */
+hydro_run* hydro_run_create(int dim, const size_t* nb, const double* D, double C_min, double dt0, double max_t,
+                            size_t max_it, int precision, const int* bc);
+
+/* Enqueue up to n_steps steps on `stream`, alternating between the (a) and (b) state
+ * buffers starting from whichever holds the current state (a before the first step).
+ * Each step fills ghosts then updates.  Steps after the loop ends are no-ops. */
+int hydro_run_advance(hydro_run* run, void* rhoE_a, void* mom_a, void* rhoE_b, void* mom_b, size_t n_steps,
+                      void* stream);
+
+/* Wait for `stream` and report the loop state: completed iterations, t, the current Dt,
+ * whether the loop has ended, and which buffer holds the current state (0 = a, 1 = b).
+ * Returns HYDRO_ERR_NONFINITE if a Dt came out NaN or infinite. */
+int hydro_run_status(hydro_run* run, size_t* it, double* t, double* dt, int* done, int* current, void* stream);
+
+/* Copy recorded (t, Dt) pairs [from, from+count) to host arrays (either may be NULL). */
+int hydro_run_trajectory(hydro_run* run, size_t from, size_t count, double* t_out, double* dt_out, void* stream);
+
+void hydro_run_destroy(hydro_run* run);
+
+/* ---------------------------------------------------------------------------------
+ * Initial condition.
+ * ------------------------------------------------------------------------------- */
+
+/* Blast initial condition on the device.  Replaces set_init_conditions_diffusion
+ * (hydro2d/main.cxx:280-338, hydro3d/main.cxx:296-363) plus the host->device copy of the
+ * initial state (hydro2d/main.cxx:650-652).  Writes the interior cells of a (sub)domain
+ * of interior size nb[dim] whose first interior cell has global 0-based index base[dim]
+ * (NULL = origin) in a global grid NB[dim]; ghost cells are not touched.  rho = rho_in /
+ * rho_out, E = p_in / (gamma-1) or p_out / (gamma-1), zero momentum.  ic_mode 0: the
+ * reference disc / sphere (centre (NB+1)/2, radius min(centre)/3); ic_mode 1: radius =
+ * radius_frac * min(NB + 1).  *min_dt receives min over cells of C_min / (c_s + |u|). */
+int hydro_ic(double* d_rhoE, double* d_mom, int dim, const size_t* nb, const size_t* base, const size_t* NB,
+             double rho_in, double rho_out, double p_in, double p_out, int ic_mode, double radius_frac, double C_min,
+             double gamma, double* min_dt, void* stream);
+int hydro_ic_f32(float* d_rhoE, float* d_mom, int dim, const size_t* nb, const size_t* base, const size_t* NB,
+                 float rho_in, float rho_out, float p_in, float p_out, int ic_mode, double radius_frac, float C_min,
+                 float gamma, float* min_dt, void* stream);
+
+/* ---------------------------------------------------------------------------------
+ * Diagnostics.
+ * ------------------------------------------------------------------------------- */
+
+/* Total mass (sum of interior rho) computed on the device with a compensated FP64
+ * reduction; the reference computes it on the host with a compensated long double sum
+ * (hydro2d/main.cxx:237-265).  nb_z = 0 for 2D.  precision 64 or 32. */
+int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision, double* mass,
+                     void* stream);
+
+const char* hydro_last_error(void);
+int hydro_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYDRO_HIP_H_ */

@@ -1,0 +1,269 @@
+"""MI355X-native first-order finite-volume compressible Euler solver (2D / 3D).
+
+Python front end of libhydro_hip.so, whose C ABI is declared in include/hydro_hip.h.
+The compute path is the hand-written HIP code of csrc/; PyTorch only provides device
+memory and streams.  There is no CPU fallback: if the library is missing or no GPU is
+visible, the calls below raise.
+
+State tensors use the reference layout (hydro2d/main.cxx:118-128,
+hydro3d/main.cxx:122-133): 2D rhoE, uv of shape (NB_X+2, NB_Y+2, 2); 3D rhoE of
+shape (NB_X+2, NB_Y+2, NB_Z+2, 2) and uvw (..., 3); contiguous, float64 (float32 for
+the USE_FLOAT form), one ghost layer.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import subprocess
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libhydro_hip.so"
+
+BC_PERIODIC = 0
+BC_REFLECTIVE = 1
+
+GAMMA = 1.4
+K = 1.1
+
+_lib = None
+
+
+def build(jobs: int = 8) -> None:
+    """Compile the HIP library for gfx950 in-tree (hipcc via make)."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", str(HERE)], check=True)
+
+
+class HydroError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """Load libhydro_hip.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise HydroError(f"{LIB_PATH} is not built; run __graft_entry__.build() or make -C {HERE}")
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L: C.CDLL) -> None:
+    vp, sz, i32, d, f = C.c_void_p, C.c_size_t, C.c_int, C.c_double, C.c_float
+    L.launcher.argtypes = [vp] * 5 + [d] * 5 + [sz, sz, d, d, vp]
+    L.launcher.restype = d
+    L.launcher3d.argtypes = [vp] * 5 + [d] * 5 + [sz, sz, sz, d, d, d, vp]
+    L.launcher3d.restype = d
+    L.launcher_f32.argtypes = [vp] * 5 + [f] * 5 + [sz, sz, f, f, vp]
+    L.launcher_f32.restype = d
+    L.launcher3d_f32.argtypes = [vp] * 5 + [f] * 5 + [sz, sz, sz, f, f, f, vp]
+    L.launcher3d_f32.restype = d
+    L.hydro_step2d.argtypes = [vp] * 5 + [d] * 5 + [sz, sz, d, d, i32, vp]
+    L.hydro_step3d.argtypes = [vp] * 5 + [d] * 5 + [sz, sz, sz, d, d, d, i32, i32, vp]
+    L.hydro_ghost_fill2d.argtypes = [vp, vp, sz, sz, i32, i32, vp]
+    L.hydro_ghost_fill3d.argtypes = [vp, vp, sz, sz, sz, i32, i32, i32, vp]
+    L.hydro_ghost_fill2d_f32.argtypes = [vp, vp, sz, sz, i32, i32, vp]
+    L.hydro_ghost_fill3d_f32.argtypes = [vp, vp, sz, sz, sz, i32, i32, i32, vp]
+    L.hydro_ic.argtypes = [vp, vp, i32, vp, vp, vp, d, d, d, d, i32, d, d, d, vp, vp]
+    L.hydro_ic_f32.argtypes = [vp, vp, i32, vp, vp, vp, f, f, f, f, i32, d, f, f, vp, vp]
+    L.hydro_run_create.argtypes = [i32, vp, vp, d, d, d, sz, i32, vp]
+    L.hydro_run_create.restype = vp
+    L.hydro_run_advance.argtypes = [vp, vp, vp, vp, vp, sz, vp]
+    L.hydro_run_status.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    L.hydro_run_trajectory.argtypes = [vp, sz, sz, vp, vp, vp]
+    L.hydro_run_destroy.argtypes = [vp]
+    L.hydro_run_destroy.restype = None
+    L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
+    L.hydro_last_error.restype = C.c_char_p
+    L.hydro_version.restype = i32
+
+
+def last_error() -> str:
+    return lib().hydro_last_error().decode()
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise HydroError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _stream(stream) -> C.c_void_p:
+    if stream is None:
+        import torch
+
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _p(t) -> C.c_void_p:
+    if not t.is_cuda:
+        raise HydroError("device tensor expected (the solver has no CPU path)")
+    if not t.is_contiguous():
+        raise HydroError("contiguous tensor expected")
+    return C.c_void_p(t.data_ptr())
+
+
+def _is64(t) -> bool:
+    import torch
+
+    if t.dtype == torch.float64:
+        return True
+    if t.dtype == torch.float32:
+        return False
+    raise HydroError(f"float64 or float32 state expected, got {t.dtype}")
+
+
+def _sizes(a) -> C.Array:
+    return (C.c_size_t * len(a))(*[int(x) for x in a])
+
+
+# ----------------------------------------------------------------------------- physics
+
+def constants(nb, C_cfl: float, dtype: str = "f64") -> dict:
+    """gamma, gamma-1, 1/(gamma-1), K, DX.., C_min as hydro2d/main.cxx:482-484, 595-607."""
+    import numpy as np
+
+    t = np.float64 if dtype == "f64" else np.float32
+    one = t(1)
+    gamma = t(GAMMA)
+    gmo = gamma - one
+    D = [one / t(n + 1) for n in nb]
+    ms = D[0]
+    for v in D[1:]:
+        ms = v if v < ms else ms
+    return {"gamma": gamma, "gmo": gmo, "divgamma": one / gmo, "K": t(K), "D": D, "C_min": t(C_cfl) * ms}
+
+
+# ----------------------------------------------------------------------------- kernels
+
+def launcher(rhoE, uv, rhoE_next, uv_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, stream=None):
+    """The drop-in 2D launcher (hydro2d/kernel.hpp:63-68): one synchronous step."""
+    nb_x, nb_y = rhoE.shape[0] - 2, rhoE.shape[1] - 2
+    fn = lib().launcher if _is64(rhoE) else lib().launcher_f32
+    us = fn(_p(rhoE), _p(uv), _p(rhoE_next), _p(uv_next), _p(Dt_next), C_min, gamma, gmo, divgamma, Kc, nb_x, nb_y,
+            DtDx, DtDy, _stream(stream))
+    if us < 0:
+        raise HydroError(f"launcher failed: {last_error()}")
+    return us
+
+
+def launcher3d(rhoE, uvw, rhoE_next, uvw_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, DtDz,
+               stream=None):
+    """The drop-in 3D launcher (hydro3d/kernel.hpp:64-69)."""
+    nb = [s - 2 for s in rhoE.shape[:3]]
+    fn = lib().launcher3d if _is64(rhoE) else lib().launcher3d_f32
+    us = fn(_p(rhoE), _p(uvw), _p(rhoE_next), _p(uvw_next), _p(Dt_next), C_min, gamma, gmo, divgamma, Kc, *nb, DtDx,
+            DtDy, DtDz, _stream(stream))
+    if us < 0:
+        raise HydroError(f"launcher3d failed: {last_error()}")
+    return us
+
+
+def step2d(rhoE, uv, rhoE_next, uv_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, bc_y=BC_REFLECTIVE,
+           stream=None):
+    nb_x, nb_y = rhoE.shape[0] - 2, rhoE.shape[1] - 2
+    _check(lib().hydro_step2d(_p(rhoE), _p(uv), _p(rhoE_next), _p(uv_next), _p(Dt_next), C_min, gamma, gmo, divgamma,
+                              Kc, nb_x, nb_y, DtDx, DtDy, bc_y, _stream(stream)), "hydro_step2d")
+
+
+def step3d(rhoE, uvw, rhoE_next, uvw_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, DtDz,
+           bc_y=BC_REFLECTIVE, bc_z=BC_REFLECTIVE, stream=None):
+    nb = [s - 2 for s in rhoE.shape[:3]]
+    _check(lib().hydro_step3d(_p(rhoE), _p(uvw), _p(rhoE_next), _p(uvw_next), _p(Dt_next), C_min, gamma, gmo, divgamma,
+                              Kc, *nb, DtDx, DtDy, DtDz, bc_y, bc_z, _stream(stream)), "hydro_step3d")
+
+
+def ghost_fill2d(rhoE, uv, bc_x=BC_PERIODIC, bc_y=BC_REFLECTIVE, stream=None):
+    nb_x, nb_y = rhoE.shape[0] - 2, rhoE.shape[1] - 2
+    fn = lib().hydro_ghost_fill2d if _is64(rhoE) else lib().hydro_ghost_fill2d_f32
+    _check(fn(_p(rhoE), _p(uv), nb_x, nb_y, bc_x, bc_y, _stream(stream)), "hydro_ghost_fill2d")
+
+
+def ghost_fill3d(rhoE, uvw, bc_x=BC_PERIODIC, bc_y=BC_REFLECTIVE, bc_z=BC_REFLECTIVE, stream=None):
+    nb = [s - 2 for s in rhoE.shape[:3]]
+    fn = lib().hydro_ghost_fill3d if _is64(rhoE) else lib().hydro_ghost_fill3d_f32
+    _check(fn(_p(rhoE), _p(uvw), *nb, bc_x, bc_y, bc_z, _stream(stream)), "hydro_ghost_fill3d")
+
+
+def ic(rhoE, mom, C_min, NB=None, base=None, ic_mode=0, radius_frac=0.0, rho_in=1.0, rho_out=1.2, p_in=10.0, p_out=0.1,
+       stream=None):
+    """Blast IC on the device (hydro2d/main.cxx:280-338); returns the initial Dt."""
+    import numpy as np
+
+    dim = rhoE.dim() - 1
+    nb = [s - 2 for s in rhoE.shape[:dim]]
+    NB = nb if NB is None else NB
+    is64 = _is64(rhoE)
+    out = (C.c_double if is64 else C.c_float)()
+    fn = lib().hydro_ic if is64 else lib().hydro_ic_f32
+    t = np.float64 if is64 else np.float32
+    _check(fn(_p(rhoE), _p(mom), dim, _sizes(nb), _sizes(base) if base is not None else None, _sizes(NB),
+              t(rho_in), t(rho_out), t(p_in), t(p_out), ic_mode, radius_frac, t(C_min), t(GAMMA), C.byref(out),
+              _stream(stream)), "hydro_ic")
+    return t(out.value)
+
+
+def total_mass(rhoE, stream=None) -> float:
+    dim = rhoE.dim() - 1
+    nb = [s - 2 for s in rhoE.shape[:dim]]
+    m = C.c_double()
+    _check(lib().hydro_total_mass(_p(rhoE), nb[0], nb[1], nb[2] if dim == 3 else 0, 64 if _is64(rhoE) else 32,
+                                  C.byref(m), _stream(stream)), "hydro_total_mass")
+    return m.value
+
+
+# ----------------------------------------------------------------------------- resident run
+
+class Run:
+    """The step loop of hydro2d/main.cxx:686-815 with Dt/t/it resident on the device."""
+
+    def __init__(self, nb, D, C_min, dt0, max_t, max_it, precision=64, bc=None):
+        self.dim = len(nb)
+        bcs = list(bc) if bc is not None else [BC_PERIODIC] + [BC_REFLECTIVE] * (self.dim - 1)
+        self._bc = (C.c_int * 3)(*(bcs + [0] * (3 - len(bcs))))
+        self.max_it = int(max_it)
+        h = lib().hydro_run_create(self.dim, _sizes(nb), (C.c_double * 3)(*[float(x) for x in D]), float(C_min),
+                                   float(dt0), float(max_t), self.max_it, precision, self._bc)
+        if not h:
+            raise HydroError(f"hydro_run_create failed: {last_error()}")
+        self._h = C.c_void_p(h)
+
+    def advance(self, a, b, n_steps, stream=None):
+        _check(lib().hydro_run_advance(self._h, _p(a[0]), _p(a[1]), _p(b[0]), _p(b[1]), int(n_steps), _stream(stream)),
+               "hydro_run_advance")
+
+    def status(self, stream=None) -> dict:
+        it, t, dt, done, cur = C.c_size_t(), C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        rc = lib().hydro_run_status(self._h, C.byref(it), C.byref(t), C.byref(dt), C.byref(done), C.byref(cur),
+                                    _stream(stream))
+        st = {"it": it.value, "t": t.value, "dt": dt.value, "done": bool(done.value), "current": cur.value, "rc": rc}
+        if rc != 0:
+            st["error"] = last_error()
+        return st
+
+    def trajectory(self, n, stream=None):
+        import numpy as np
+
+        tt = np.zeros(max(n, 1), np.float64)
+        td = np.zeros(max(n, 1), np.float64)
+        _check(lib().hydro_run_trajectory(self._h, 0, int(n), tt.ctypes.data_as(C.c_void_p),
+                                          td.ctypes.data_as(C.c_void_p), _stream(stream)), "hydro_run_trajectory")
+        return tt[:n], td[:n]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hydro_run_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def library_path() -> str:
+    return os.fspath(LIB_PATH)

@@ -1,0 +1,499 @@
+// 2D first-order finite-volume Euler step for gfx950 (MI355X).
+//
+// Reference hot path: kernel_hydro_fvm (hydro2d/kernel.cxx:132-319) behind `launcher`
+// (hydro2d/kernel.cxx:331-367), called once per step from hydro2d/main.cxx:734-736.
+//
+// Work decomposition (the FPGA line buffer becomes registers of a marching wavefront):
+//   * a wave (64 lanes) owns a window of 62 consecutive Y columns plus one halo column
+//     on each side (lane l <-> column j = 62*w + l) and marches along X over a strip
+//     of `rows` rows, one row per iteration;
+//   * the X face between rows i and i+1 is computed once and carried in registers to
+//     the next iteration (it is fx2 of row i and fx1 of row i+1);
+//   * the Y face between columns j and j+1 is computed once by lane j and handed to
+//     lane j+1 with a cross-lane shuffle (fy2 of column j, fy1 of column j+1);
+//   * primitives and the impedance sqrt(gamma*p*rho) are computed once per cell.
+// Each wave is independent (no LDS, no barrier in the loop); halo columns cost 2/64
+// of the lanes.  Loads and stores are 16-byte (rho,E) and (rho*u, rho*v) pairs, so a
+// wave row is one contiguous 1 KiB stream per array.
+//
+// Dt: every lane folds the signal speed of its updated cells into a running max
+// (as an order key); waves reduce by shuffles, workgroups by one 64-bit atomic max,
+// and the last workgroup to finish (ticket counter) computes Dt = C_min / max_speed.
+#include <hip/hip_runtime.h>
+
+#include "../../include/hydro_hip.h"
+#include "hydro_device.hpp"
+#include "hydro_kernels.hpp"
+#include "hydro_runtime.hpp"
+
+namespace hydro {
+
+// Reduction state of the plain (non-resident) launchers, one per device and precision
+// ([0] FP64, [1] FP32); its device address is looked up with hipGetSymbolAddress.
+__device__ StepReduce g_red2d[2];
+
+namespace {
+
+constexpr int kOwned = kWave - 2;  // owned columns per wave window
+constexpr int kWavesPerBlock = 4;
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+
+struct Geom2 {
+    int nb_x, nb_y;
+    long long ys;  // nb_y + 2
+    int rows;      // rows per strip
+    int nwin;      // column windows
+    int nwb;       // workgroups across Y (nwin / kWavesPerBlock, rounded up)
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2): give each
+// XCD a contiguous range of logical blocks so that neighbouring column windows, which
+// share halo columns, run on the same L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
+{
+    constexpr unsigned X = 8;
+    const unsigned q = n / X, r = n % X;
+    const unsigned xcd = b % X, k = b / X;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+template <typename T>
+struct Consts {
+    T C_min, gamma, gmo, divgamma, K;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_cell(const typename Vec2<T>::type* __restrict__ re,
+                                          const typename Vec2<T>::type* __restrict__ mo, long long n,
+                                          typename Vec2<T>::type& a, typename Vec2<T>::type& b)
+{
+    a = re[n];
+    b = mo[n];
+}
+
+template <typename T, bool WALL_Y, bool RESIDENT>
+__global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rhoE, const T* __restrict__ uv,
+                                                       T* __restrict__ rhoE_next, T* __restrict__ uv_next, Geom2 g,
+                                                       Consts<T> c, T DtDx_arg, T DtDy_arg, StepReduce* red,
+                                                       StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt)
+{
+    using V2 = typename Vec2<T>::type;
+    T DtDx = DtDx_arg, DtDy = DtDy_arg;
+    if constexpr (RESIDENT) {
+        if (ctl->done) return;  // uniform: the loop has ended, this step is a no-op
+        const T dt = static_cast<T>(ctl->dt);
+        DtDx = dt / static_cast<T>(ctl->D[0]);  // hydro2d/main.cxx:703-704
+        DtDy = dt / static_cast<T>(ctl->D[1]);
+    }
+    const V2* __restrict__ R2 = reinterpret_cast<const V2*>(rhoE);
+    const V2* __restrict__ M2 = reinterpret_cast<const V2*>(uv);
+    V2* __restrict__ RN2 = reinterpret_cast<V2*>(rhoE_next);
+    V2* __restrict__ MN2 = reinterpret_cast<V2*>(uv_next);
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int wb = static_cast<int>(lb % static_cast<unsigned>(g.nwb));
+    const int strip = static_cast<int>(lb / static_cast<unsigned>(g.nwb));
+    const int win = wb * kWavesPerBlock + wv;
+    const int j = win * kOwned + lane;
+    const bool owned = (win < g.nwin) && lane >= 1 && lane <= kOwned && j <= g.nb_y;
+    const int jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;  // clamped so every load is in bounds
+    const int i0 = 1 + strip * g.rows;
+    const int i1 = min(i0 + g.rows - 1, g.nb_x);
+
+    uint64_t key = 0;
+    if (win < g.nwin) {
+        // Row i0-1 and i0 primitives; X face (i0-1, i0).
+        V2 ra, ma, rc, mc, rn, mn;
+        load_cell<T>(R2, M2, static_cast<long long>(i0 - 1) * g.ys + jl, ra, ma);
+        load_cell<T>(R2, M2, static_cast<long long>(i0) * g.ys + jl, rc, mc);
+        load_cell<T>(R2, M2, static_cast<long long>(i0 + 1) * g.ys + jl, rn, mn);
+        Prim2<T> Pm, Pc;  // X frame: un = u, ut = v
+        Pm.r = ra.x;
+        cons_to_prim2<T>(ra.x, ma.x, ma.y, ra.y, c.gmo, c.gamma, Pm.un, Pm.ut, Pm.p, Pm.a);
+        Pc.r = rc.x;
+        cons_to_prim2<T>(rc.x, mc.x, mc.y, rc.y, c.gmo, c.gamma, Pc.un, Pc.ut, Pc.p, Pc.a);
+        T fx1_0, fx1_1, fx1_2, fx1_3;
+        face_flux2<T>(Pm, Pc, c.K, c.divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
+
+        for (int i = i0; i <= i1; ++i) {
+            // Prefetch row i+2 (clamped to the last row this strip reads).
+            V2 rnn, mnn;
+            const int ipf = min(i + 2, i1 + 1);
+            load_cell<T>(R2, M2, static_cast<long long>(ipf) * g.ys + jl, rnn, mnn);
+
+            // Y faces of row i, in the (rho, v, u, p) frame (kernel.cxx:213-251).
+            Prim2<T> Sy, Ny;
+            Sy.r = Pc.r;
+            Sy.un = Pc.ut;
+            Sy.ut = Pc.un;
+            Sy.p = Pc.p;
+            Sy.a = Pc.a;
+            Ny.r = from_next_lane(Sy.r);
+            Ny.un = from_next_lane(Sy.un);
+            Ny.ut = from_next_lane(Sy.ut);
+            Ny.p = from_next_lane(Sy.p);
+            Ny.a = from_next_lane(Sy.a);
+            T fyR0, fyR1, fyR2, fyR3;
+            face_flux2<T>(Sy, Ny, c.K, c.divgamma, fyR0, fyR1, fyR2, fyR3);
+            if constexpr (WALL_Y) {
+                // Reflective walls: the outside state is the cell itself with v negated.
+                // Lane of column nb_y evaluates flux(c, mirror(c)) (kernel.cxx:233-251);
+                // the lane of ghost column 0 evaluates flux(mirror(c1), c1), which
+                // column 1 receives as its fy1 (kernel.cxx:213-230).
+                const bool bottom = (j == g.nb_y);
+                const bool top = (j == 0);
+                if (bottom || top) {
+                    Prim2<T> L = bottom ? Sy : Ny;
+                    Prim2<T> R = L;
+                    if (bottom) R.un = -Sy.un;
+                    else L.un = -Ny.un;
+                    face_flux2<T>(L, R, c.K, c.divgamma, fyR0, fyR1, fyR2, fyR3);
+                }
+            }
+            const T fyL0 = from_prev_lane(fyR0);
+            const T fyL1 = from_prev_lane(fyR1);
+            const T fyL2 = from_prev_lane(fyR2);
+            const T fyL3 = from_prev_lane(fyR3);
+
+            // Row i+1 primitives and the X face (i, i+1).
+            Prim2<T> Pn;
+            Pn.r = rn.x;
+            cons_to_prim2<T>(rn.x, mn.x, mn.y, rn.y, c.gmo, c.gamma, Pn.un, Pn.ut, Pn.p, Pn.a);
+            T fx2_0, fx2_1, fx2_2, fx2_3;
+            face_flux2<T>(Pc, Pn, c.K, c.divgamma, fx2_0, fx2_1, fx2_2, fx2_3);
+
+            // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the
+            // (rho, v, u, E) frame, so rho*u takes component 2 and rho*v component 1.
+            const T x_r = DtDx * (fx1_0 - fx2_0);
+            const T x_u = DtDx * (fx1_1 - fx2_1);
+            const T x_v = DtDx * (fx1_2 - fx2_2);
+            const T x_E = DtDx * (fx1_3 - fx2_3);
+            const T y_r = DtDy * (fyL0 - fyR0);
+            const T y_u = DtDy * (fyL2 - fyR2);
+            const T y_v = DtDy * (fyL1 - fyR1);
+            const T y_E = DtDy * (fyL3 - fyR3);
+            const T nr = rc.x + x_r + y_r;
+            const T nu = mc.x + x_u + y_u;
+            const T nv = mc.y + x_v + y_v;
+            const T nE = rc.y + x_E + y_E;
+            if (owned) {
+                const long long n = static_cast<long long>(i) * g.ys + j;
+                V2 o1, o2;
+                o1.x = nr;
+                o1.y = nE;
+                o2.x = nu;
+                o2.y = nv;
+                RN2[n] = o1;
+                MN2[n] = o2;
+                key = umax64(key, speed_key(static_cast<double>(signal_speed2<T>(nr, nu, nv, nE, c.gmo, c.gamma))));
+            }
+
+            // Advance the march.
+            Pc = Pn;
+            rc = rn;
+            mc = mn;
+            rn = rnn;
+            mn = mnn;
+            fx1_0 = fx2_0;
+            fx1_1 = fx2_1;
+            fx1_2 = fx2_2;
+            fx1_3 = fx2_3;
+        }
+    }
+
+    const uint64_t wkey = wave_max_u64(key);
+    if (reduce_and_elect(red, wkey, gridDim.x)) {
+        const T dtn = finalize_dt<T>(red, c.C_min);
+        if constexpr (RESIDENT) {
+            const unsigned long long it = ctl->it;
+            if (traj_t) traj_t[it] = ctl->t;
+            if (traj_dt) traj_dt[it] = dtn;
+            const double t_new = ctl->t + static_cast<double>(dtn);  // hydro2d/main.cxx:813-814
+            ctl->dt = static_cast<double>(dtn);
+            ctl->t = t_new;
+            ctl->it = it + 1;
+            if (!isfinite(static_cast<double>(dtn))) {
+                ctl->nonfinite = 1u;
+                ctl->done = 1u;
+            }
+            if (!(t_new < ctl->max_t && it + 1 < ctl->max_it)) ctl->done = 1u;
+        } else {
+            *Dt_next = dtn;
+        }
+    }
+}
+
+// Ghost fill: X planes (periodic copy or reflective mirror), hydro2d/main.cxx:349-378.
+template <typename T>
+__global__ void ghost_x2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long long nb_x, long long ys, int bc_x,
+                                 const StepControl* ctl)
+{
+    using V2 = typename Vec2<T>::type;
+    if (ctl && ctl->done) return;
+    V2* R2 = reinterpret_cast<V2*>(rhoE);
+    V2* M2 = reinterpret_cast<V2*>(uv);
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 2 * ys) return;
+    const int side = t >= ys;  // 0: plane 0, 1: plane nb_x+1
+    const long long j = side ? t - ys : t;
+    long long dst, src;
+    if (bc_x == HYDRO_BC_PERIODIC) {
+        dst = side ? (nb_x + 1) * ys + j : j;
+        src = side ? ys + j : nb_x * ys + j;
+    } else {
+        dst = side ? (nb_x + 1) * ys + j : j;
+        src = side ? nb_x * ys + j : ys + j;
+    }
+    R2[dst] = R2[src];
+    V2 m = M2[src];
+    if (bc_x != HYDRO_BC_PERIODIC) m.x = -m.x;
+    M2[dst] = m;
+}
+
+// Periodic Y ghosts in every plane (after the X planes): column 0 <- column nb_y,
+// column nb_y+1 <- column 1.
+template <typename T>
+__global__ void ghost_y2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long long nb_x, long long nb_y,
+                                 const StepControl* ctl)
+{
+    using V2 = typename Vec2<T>::type;
+    if (ctl && ctl->done) return;
+    V2* R2 = reinterpret_cast<V2*>(rhoE);
+    V2* M2 = reinterpret_cast<V2*>(uv);
+    const long long ys = nb_y + 2;
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 2 * (nb_x + 2)) return;
+    const long long i = t >> 1;
+    const int side = static_cast<int>(t & 1);
+    const long long dst = i * ys + (side ? nb_y + 1 : 0);
+    const long long src = i * ys + (side ? 1 : nb_y);
+    R2[dst] = R2[src];
+    M2[dst] = M2[src];
+}
+
+int g_rows_2d = 64;  // rows per strip (tunable, hydro_tune_set)
+
+template <typename T>
+int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
+                  T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,
+                  hipStream_t s)
+{
+    if (!rhoE || !uv || !rhoE_next || !uv_next || (!ctl && !Dt_next)) {
+        set_error("hydro 2D step: null pointer argument");
+        return HYDRO_ERR_ARG;
+    }
+    if (nb_x < 1 || nb_y < 1 || nb_x > (1u << 30) || nb_y > (1u << 30)) {
+        set_error("hydro 2D step: interior sizes must be in [1, 2^30], got %zu x %zu", nb_x, nb_y);
+        return HYDRO_ERR_ARG;
+    }
+    if (bc_y != HYDRO_BC_PERIODIC && bc_y != HYDRO_BC_REFLECTIVE) {
+        set_error("hydro 2D step: unknown bc_y %d", bc_y);
+        return HYDRO_ERR_ARG;
+    }
+    Geom2 g;
+    g.nb_x = static_cast<int>(nb_x);
+    g.nb_y = static_cast<int>(nb_y);
+    g.ys = static_cast<long long>(nb_y) + 2;
+    g.rows = g_rows_2d;
+    g.nwin = static_cast<int>((nb_y + kOwned - 1) / kOwned);
+    g.nwb = (g.nwin + kWavesPerBlock - 1) / kWavesPerBlock;
+    const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
+    const long long nblocks = nstrips * g.nwb;
+    if (nblocks > 0x7fffffffLL) {
+        set_error("hydro 2D step: grid too large (%lld workgroups)", nblocks);
+        return HYDRO_ERR_ARG;
+    }
+    const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock);
+    const bool wall = bc_y == HYDRO_BC_REFLECTIVE;
+    if (ctl) {
+        if (wall)
+            hipLaunchKernelGGL((fvm2d_kernel<T, true, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+                               T(0), T(0), red, ctl, Dt_next, traj_t, traj_dt);
+        else
+            hipLaunchKernelGGL((fvm2d_kernel<T, false, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+                               T(0), T(0), red, ctl, Dt_next, traj_t, traj_dt);
+    } else {
+        if (wall)
+            hipLaunchKernelGGL((fvm2d_kernel<T, true, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+                               DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
+        else
+            hipLaunchKernelGGL((fvm2d_kernel<T, false, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+                               DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
+    }
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    return 0;
+}
+
+template <typename T>
+int launch_ghost2d(T* rhoE, T* uv, size_t nb_x, size_t nb_y, int bc_x, int bc_y, const StepControl* ctl,
+                   hipStream_t s)
+{
+    if (!rhoE || !uv || nb_x < 1 || nb_y < 1) {
+        set_error("hydro ghost fill 2D: bad arguments");
+        return HYDRO_ERR_ARG;
+    }
+    if ((bc_x != HYDRO_BC_PERIODIC && bc_x != HYDRO_BC_REFLECTIVE) ||
+        (bc_y != HYDRO_BC_PERIODIC && bc_y != HYDRO_BC_REFLECTIVE)) {
+        set_error("hydro ghost fill 2D: unknown boundary mode (%d, %d)", bc_x, bc_y);
+        return HYDRO_ERR_ARG;
+    }
+    const long long ys = static_cast<long long>(nb_y) + 2;
+    const unsigned nbx = static_cast<unsigned>((2 * ys + 255) / 256);
+    hipLaunchKernelGGL((ghost_x2d_kernel<T>), dim3(nbx), dim3(256), 0, s, rhoE, uv, static_cast<long long>(nb_x), ys,
+                       bc_x, ctl);
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    if (bc_y == HYDRO_BC_PERIODIC) {
+        const unsigned nby = static_cast<unsigned>((2 * (static_cast<long long>(nb_x) + 2) + 255) / 256);
+        hipLaunchKernelGGL((ghost_y2d_kernel<T>), dim3(nby), dim3(256), 0, s, rhoE, uv, static_cast<long long>(nb_x),
+                           static_cast<long long>(nb_y), ctl);
+        HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    }
+    return 0;
+}
+
+
+StepReduce* plain_reduce_state(int which)
+{
+    static StepReduce* addr[64] = {};
+    const int dev = current_device();
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (!addr[dev]) {
+        void* p = nullptr;
+        HYDRO_HIP_OR(hipGetSymbolAddress(&p, HIP_SYMBOL(g_red2d)), nullptr);
+        addr[dev] = static_cast<StepReduce*>(p);
+    }
+    return addr[dev] + which;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+EventPair* timing_events()
+{
+    static thread_local EventPair ev[64];
+    const int dev = current_device();
+    if (dev < 0 || dev >= 64) return nullptr;
+    EventPair& e = ev[dev];
+    if (!e.a) {
+        HYDRO_HIP_OR(hipEventCreate(&e.a), nullptr);
+        HYDRO_HIP_OR(hipEventCreate(&e.b), nullptr);
+    }
+    return &e;
+}
+
+template <typename T>
+double timed_launcher2d(const T* d_rhoE, const T* d_uv, T* d_rhoE_next, T* d_uv_next, T* Dt_next, T C_min, T gamma,
+                        T gamma_minus_one, T divgamma, T K, size_t NB_X, size_t NB_Y, T DtDx, T DtDy, void* stream)
+{
+    clear_error();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StepReduce* red = plain_reduce_state(sizeof(T) == 8 ? 0 : 1);
+    EventPair* ev = timing_events();
+    if (!red || !ev) return HYDRO_ERR_HIP;
+    const Consts<T> c{ C_min, gamma, gamma_minus_one, divgamma, K };
+    HYDRO_HIP_OR(hipEventRecord(ev->a, s), HYDRO_ERR_HIP);
+    int rc = launch_step2d<T>(d_rhoE, d_uv, d_rhoE_next, d_uv_next, c, NB_X, NB_Y, DtDx, DtDy, HYDRO_BC_REFLECTIVE,
+                              red, nullptr, Dt_next, nullptr, nullptr, s);
+    if (rc) return rc;
+    HYDRO_HIP_OR(hipEventRecord(ev->b, s), HYDRO_ERR_HIP);
+    HYDRO_HIP_OR(hipEventSynchronize(ev->b), HYDRO_ERR_HIP);
+    float ms = 0.f;
+    HYDRO_HIP_OR(hipEventElapsedTime(&ms, ev->a, ev->b), HYDRO_ERR_HIP);
+    return static_cast<double>(ms) * 1e3;
+}
+
+}  // namespace
+
+// ---- entry points used by the resident run loop and the batched forms -------------
+
+int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
+                    int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
+                    void* traj_dt, hipStream_t s)
+{
+    if (precision == 64) {
+        const Consts<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
+        return launch_step2d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
+                                     static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c, nb_x, nb_y,
+                                     0.0, 0.0, bc_y, red, ctl, nullptr, traj_t, static_cast<double*>(traj_dt), s);
+    }
+    const Consts<float> c{ static_cast<float>(consts[0]), static_cast<float>(consts[1]),
+                           static_cast<float>(consts[2]), static_cast<float>(consts[3]),
+                           static_cast<float>(consts[4]) };
+    return launch_step2d<float>(static_cast<const float*>(rhoE), static_cast<const float*>(mom),
+                                static_cast<float*>(rhoE_next), static_cast<float*>(mom_next), c, nb_x, nb_y, 0.f, 0.f,
+                                bc_y, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s);
+}
+
+int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,
+                const StepControl* ctl, hipStream_t s)
+{
+    if (precision == 64)
+        return launch_ghost2d<double>(static_cast<double*>(rhoE), static_cast<double*>(mom), nb_x, nb_y, bc_x, bc_y,
+                                      ctl, s);
+    return launch_ghost2d<float>(static_cast<float*>(rhoE), static_cast<float*>(mom), nb_x, nb_y, bc_x, bc_y, ctl, s);
+}
+
+void set_rows_2d(int rows)
+{
+    if (rows >= 4 && rows <= 4096) g_rows_2d = rows;
+}
+
+}  // namespace hydro
+
+using namespace hydro;
+
+extern "C" double launcher(const double* d_rhoE, const double* d_uv, double* d_rhoE_next, double* d_uv_next,
+                           double* Dt_next, double C_min, double gamma, double gamma_minus_one, double divgamma,
+                           double K, size_t NB_X, size_t NB_Y, double DtDx, double DtDy, void* stream)
+{
+    return timed_launcher2d<double>(d_rhoE, d_uv, d_rhoE_next, d_uv_next, Dt_next, C_min, gamma, gamma_minus_one,
+                                    divgamma, K, NB_X, NB_Y, DtDx, DtDy, stream);
+}
+
+extern "C" double launcher_f32(const float* d_rhoE, const float* d_uv, float* d_rhoE_next, float* d_uv_next,
+                               float* Dt_next, float C_min, float gamma, float gamma_minus_one, float divgamma,
+                               float K, size_t NB_X, size_t NB_Y, float DtDx, float DtDy, void* stream)
+{
+    return timed_launcher2d<float>(d_rhoE, d_uv, d_rhoE_next, d_uv_next, Dt_next, C_min, gamma, gamma_minus_one,
+                                   divgamma, K, NB_X, NB_Y, DtDx, DtDy, stream);
+}
+
+extern "C" int hydro_step2d(const double* d_rhoE, const double* d_uv, double* d_rhoE_next, double* d_uv_next,
+                            double* Dt_next, double C_min, double gamma, double gamma_minus_one, double divgamma,
+                            double K, size_t NB_X, size_t NB_Y, double DtDx, double DtDy, int bc_y, void* stream)
+{
+    clear_error();
+    StepReduce* red = plain_reduce_state(0);
+    if (!red) return HYDRO_ERR_HIP;
+    const Consts<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };
+    return launch_step2d<double>(d_rhoE, d_uv, d_rhoE_next, d_uv_next, c, NB_X, NB_Y, DtDx, DtDy, bc_y, red, nullptr,
+                                 Dt_next, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hydro_ghost_fill2d(double* d_rhoE, double* d_uv, size_t NB_X, size_t NB_Y, int bc_x, int bc_y,
+                                  void* stream)
+{
+    clear_error();
+    return launch_ghost2d<double>(d_rhoE, d_uv, NB_X, NB_Y, bc_x, bc_y, nullptr, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hydro_ghost_fill2d_f32(float* d_rhoE, float* d_uv, size_t NB_X, size_t NB_Y, int bc_x, int bc_y,
+                                      void* stream)
+{
+    clear_error();
+    return launch_ghost2d<float>(d_rhoE, d_uv, NB_X, NB_Y, bc_x, bc_y, nullptr, static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,633 @@
+// 3D first-order finite-volume Euler step for gfx950 (MI355X).
+//
+// Reference hot path: kernel_hydro3d_fvm (hydro3d/kernel.cxx:145-407) behind the 3D
+// `launcher` (hydro3d/kernel.cxx:419-455), called from hydro3d/main.cxx:791-793.
+//
+// Work decomposition:
+//   * a workgroup covers a (Y x Z) tile of TY rows x 62 columns and marches along X
+//     over a strip of rows; it has TY + 2 waves, wave w holding Y row j0 - 1 + w
+//     (waves 0 and TY+1 are halo rows: they load and convert their row but update
+//     nothing);
+//   * within a wave, lane l holds Z column k = 62*kw + l (lanes 0 and 63 are halo
+//     columns); the Z face (k, k+1) is computed once by lane k and shuffled to lane k+1;
+//   * the Y face (j, j+1) is computed once by the wave of row j from the primitives
+//     the wave of row j+1 left in LDS, and left in LDS for the wave of row j+1
+//     (two workgroup barriers per X step, single-buffered LDS);
+//   * the X face between rows i and i+1 is carried in registers along the march.
+// rhoE is read and written as 16-byte (rho, E) pairs and uvw as 24-byte triples; lanes
+// are consecutive in Z, so every wave row is one contiguous stream per array.
+#include <hip/hip_runtime.h>
+
+#include "../../include/hydro_hip.h"
+#include "hydro_device.hpp"
+#include "hydro_kernels.hpp"
+#include "hydro_runtime.hpp"
+
+namespace hydro {
+
+// Reduction state of the plain (non-resident) launchers, one per device and precision
+// ([0] FP64, [1] FP32); its device address is looked up with hipGetSymbolAddress.
+__device__ StepReduce g_red3d[2];
+
+namespace {
+
+constexpr int kOwnedZ = kWave - 2;
+constexpr int kTY = 6;               // owned Y rows per workgroup
+constexpr int kNW = kTY + 2;         // waves per workgroup
+constexpr int kBlock3 = kNW * kWave;
+
+template <typename T>
+struct Vec2_3;
+template <>
+struct Vec2_3<double> {
+    using type = double2;
+};
+template <>
+struct Vec2_3<float> {
+    using type = float2;
+};
+
+template <typename T>
+struct Trip {
+    T x, y, z;
+};
+
+struct Geom3 {
+    int nb_x, nb_y, nb_z;
+    long long zs, zy;  // nb_z + 2, (nb_y + 2) * (nb_z + 2)
+    int rows;          // X rows per strip
+    int nzw, nyw;      // Z windows, Y tiles
+};
+
+template <typename T>
+struct Consts3 {
+    T C_min, gamma, gmo, divgamma, K;
+};
+
+__device__ __forceinline__ unsigned xcd_remap3(unsigned b, unsigned n)
+{
+    constexpr unsigned X = 8;
+    const unsigned q = n / X, r = n % X;
+    const unsigned xcd = b % X, k = b / X;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// Primitive state in the X frame: (rho, u, v, w, p, a).
+template <typename T>
+struct P6 {
+    T r, u, v, w, p, a;
+};
+
+template <typename T>
+__device__ __forceinline__ void load3(const typename Vec2_3<T>::type* __restrict__ R2,
+                                      const Trip<T>* __restrict__ M3, long long n, typename Vec2_3<T>::type& re,
+                                      Trip<T>& m)
+{
+    re = R2[n];
+    m = M3[n];
+}
+
+template <typename T>
+__device__ __forceinline__ void to_prim(const typename Vec2_3<T>::type& re, const Trip<T>& m, T gmo, T gamma, P6<T>& q)
+{
+    q.r = re.x;
+    cons_to_prim3<T>(re.x, m.x, m.y, m.z, re.y, gmo, gamma, q.u, q.v, q.w, q.p, q.a);
+}
+
+template <typename T>
+__device__ __forceinline__ Prim3<T> frame_x(const P6<T>& q)
+{
+    return Prim3<T>{ q.r, q.u, q.v, q.w, q.p, q.a };
+}
+template <typename T>
+__device__ __forceinline__ Prim3<T> frame_y(const P6<T>& q)
+{
+    return Prim3<T>{ q.r, q.v, q.u, q.w, q.p, q.a };  // hydro3d/kernel.cxx:254-256
+}
+template <typename T>
+__device__ __forceinline__ Prim3<T> frame_z(const P6<T>& q)
+{
+    return Prim3<T>{ q.r, q.w, q.v, q.u, q.p, q.a };  // hydro3d/kernel.cxx:296-298
+}
+
+template <typename T, bool WALL_Y, bool WALL_Z, bool RESIDENT>
+__global__ void __launch_bounds__(kBlock3) fvm3d_kernel(const T* __restrict__ rhoE, const T* __restrict__ uvw,
+                                                        T* __restrict__ rhoE_next, T* __restrict__ uvw_next, Geom3 g,
+                                                        Consts3<T> c, T DtDx_arg, T DtDy_arg, T DtDz_arg,
+                                                        StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t,
+                                                        T* traj_dt)
+{
+    using V2 = typename Vec2_3<T>::type;
+    // Y-frame primitives (rho, v, u, w, p, a) and Y-face fluxes of every wave, per lane.
+    __shared__ T s_prim[kNW][6][kWave];
+    __shared__ T s_flux[kNW][5][kWave];
+
+    T DtDx = DtDx_arg, DtDy = DtDy_arg, DtDz = DtDz_arg;
+    if constexpr (RESIDENT) {
+        if (ctl->done) return;
+        const T dt = static_cast<T>(ctl->dt);
+        DtDx = dt / static_cast<T>(ctl->D[0]);  // hydro3d/main.cxx:759-761
+        DtDy = dt / static_cast<T>(ctl->D[1]);
+        DtDz = dt / static_cast<T>(ctl->D[2]);
+    }
+    const V2* __restrict__ R2 = reinterpret_cast<const V2*>(rhoE);
+    const Trip<T>* __restrict__ M3 = reinterpret_cast<const Trip<T>*>(uvw);
+    V2* __restrict__ RN2 = reinterpret_cast<V2*>(rhoE_next);
+    Trip<T>* __restrict__ MN3 = reinterpret_cast<Trip<T>*>(uvw_next);
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const unsigned lb = xcd_remap3(blockIdx.x, gridDim.x);
+    const unsigned per_strip = static_cast<unsigned>(g.nzw) * static_cast<unsigned>(g.nyw);
+    const int strip = static_cast<int>(lb / per_strip);
+    const unsigned rem = lb % per_strip;
+    const int yt = static_cast<int>(rem / static_cast<unsigned>(g.nzw));
+    const int zw = static_cast<int>(rem % static_cast<unsigned>(g.nzw));
+
+    const int j = yt * kTY + w;  // row of this wave (j0 - 1 + w with j0 = 1 + yt*TY)
+    const int k = zw * kOwnedZ + lane;
+    const bool halo_wave = (w == 0) || (w == kNW - 1);
+    const bool row_in = j <= g.nb_y;
+    const bool owned = !halo_wave && row_in && lane >= 1 && lane <= kOwnedZ && k <= g.nb_z;
+    const int jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
+    const int kl = k <= g.nb_z + 1 ? k : g.nb_z + 1;
+    const long long col = static_cast<long long>(jl) * g.zs + kl;
+    const int i0 = 1 + strip * g.rows;
+    const int i1 = min(i0 + g.rows - 1, g.nb_x);
+
+    uint64_t key = 0;
+    V2 rc, rn;
+    Trip<T> mc, mn;
+    P6<T> Pc;
+    T fx1_0 = T(0), fx1_1 = T(0), fx1_2 = T(0), fx1_3 = T(0), fx1_4 = T(0);
+    load3<T>(R2, M3, static_cast<long long>(i0) * g.zy + col, rc, mc);
+    load3<T>(R2, M3, static_cast<long long>(i0 + 1) * g.zy + col, rn, mn);
+    to_prim<T>(rc, mc, c.gmo, c.gamma, Pc);
+    if (!halo_wave) {
+        V2 ra;
+        Trip<T> ma;
+        P6<T> Pm;
+        load3<T>(R2, M3, static_cast<long long>(i0 - 1) * g.zy + col, ra, ma);
+        to_prim<T>(ra, ma, c.gmo, c.gamma, Pm);
+        face_flux3<T>(frame_x(Pm), frame_x(Pc), c.K, c.divgamma, fx1_0, fx1_1, fx1_2, fx1_3, fx1_4);
+    }
+
+    for (int i = i0; i <= i1; ++i) {
+        V2 rnn;
+        Trip<T> mnn;
+        const int ipf = min(i + 2, i1 + 1);
+        load3<T>(R2, M3, static_cast<long long>(ipf) * g.zy + col, rnn, mnn);
+
+        // (a) publish this row's Y-frame primitives.
+        const Prim3<T> Sy = frame_y(Pc);
+        s_prim[w][0][lane] = Sy.r;
+        s_prim[w][1][lane] = Sy.un;
+        s_prim[w][2][lane] = Sy.ut1;
+        s_prim[w][3][lane] = Sy.ut2;
+        s_prim[w][4][lane] = Sy.p;
+        s_prim[w][5][lane] = Sy.a;
+        __syncthreads();
+
+        // (b) Y face (j, j+1), hydro3d/kernel.cxx:239-278, evaluated by the wave of row j.
+        if (w < kNW - 1) {
+            Prim3<T> Ny;
+            Ny.r = s_prim[w + 1][0][lane];
+            Ny.un = s_prim[w + 1][1][lane];
+            Ny.ut1 = s_prim[w + 1][2][lane];
+            Ny.ut2 = s_prim[w + 1][3][lane];
+            Ny.p = s_prim[w + 1][4][lane];
+            Ny.a = s_prim[w + 1][5][lane];
+            Prim3<T> L = Sy, R = Ny;
+            if constexpr (WALL_Y) {
+                if (j == g.nb_y) {  // wall below row nb_y: flux(c, mirror(c))
+                    R = Sy;
+                    R.un = -Sy.un;
+                } else if (j == 0) {  // wall above row 1: flux(mirror(c1), c1)
+                    L = Ny;
+                    L.un = -Ny.un;
+                }
+            }
+            T f0, f1, f2, f3, f4;
+            face_flux3<T>(L, R, c.K, c.divgamma, f0, f1, f2, f3, f4);
+            s_flux[w][0][lane] = f0;
+            s_flux[w][1][lane] = f1;
+            s_flux[w][2][lane] = f2;
+            s_flux[w][3][lane] = f3;
+            s_flux[w][4][lane] = f4;
+        }
+        __syncthreads();
+
+        V2 rcur = rc;
+        Trip<T> mcur = mc;
+        P6<T> Pn;
+        to_prim<T>(rn, mn, c.gmo, c.gamma, Pn);
+        if (!halo_wave) {
+            // (c) Y face above this row, from the wave of row j-1.
+            const T fy1_0 = s_flux[w - 1][0][lane];
+            const T fy1_1 = s_flux[w - 1][1][lane];
+            const T fy1_2 = s_flux[w - 1][2][lane];
+            const T fy1_3 = s_flux[w - 1][3][lane];
+            const T fy1_4 = s_flux[w - 1][4][lane];
+            const T fy2_0 = s_flux[w][0][lane];
+            const T fy2_1 = s_flux[w][1][lane];
+            const T fy2_2 = s_flux[w][2][lane];
+            const T fy2_3 = s_flux[w][3][lane];
+            const T fy2_4 = s_flux[w][4][lane];
+
+            // Z face (k, k+1) by lane k, shuffled to lane k+1, hydro3d/kernel.cxx:281-320.
+            const Prim3<T> Sz = frame_z(Pc);
+            Prim3<T> Nz;
+            Nz.r = from_next_lane(Sz.r);
+            Nz.un = from_next_lane(Sz.un);
+            Nz.ut1 = from_next_lane(Sz.ut1);
+            Nz.ut2 = from_next_lane(Sz.ut2);
+            Nz.p = from_next_lane(Sz.p);
+            Nz.a = from_next_lane(Sz.a);
+            T fz2_0, fz2_1, fz2_2, fz2_3, fz2_4;
+            face_flux3<T>(Sz, Nz, c.K, c.divgamma, fz2_0, fz2_1, fz2_2, fz2_3, fz2_4);
+            if constexpr (WALL_Z) {
+                const bool back = (k == g.nb_z);
+                const bool front = (k == 0);
+                if (back || front) {
+                    Prim3<T> L = back ? Sz : Nz;
+                    Prim3<T> R = L;
+                    if (back) R.un = -Sz.un;
+                    else L.un = -Nz.un;
+                    face_flux3<T>(L, R, c.K, c.divgamma, fz2_0, fz2_1, fz2_2, fz2_3, fz2_4);
+                }
+            }
+            const T fz1_0 = from_prev_lane(fz2_0);
+            const T fz1_1 = from_prev_lane(fz2_1);
+            const T fz1_2 = from_prev_lane(fz2_2);
+            const T fz1_3 = from_prev_lane(fz2_3);
+            const T fz1_4 = from_prev_lane(fz2_4);
+
+            // X face (i, i+1).
+            T fx2_0, fx2_1, fx2_2, fx2_3, fx2_4;
+            face_flux3<T>(frame_x(Pc), frame_x(Pn), c.K, c.divgamma, fx2_0, fx2_1, fx2_2, fx2_3, fx2_4);
+
+            // Update, hydro3d/kernel.cxx:330-358: U' = ((U + x) + y) + z.
+            const T x_r = DtDx * (fx1_0 - fx2_0);
+            const T x_u = DtDx * (fx1_1 - fx2_1);
+            const T x_v = DtDx * (fx1_2 - fx2_2);
+            const T x_w = DtDx * (fx1_3 - fx2_3);
+            const T x_E = DtDx * (fx1_4 - fx2_4);
+            const T y_r = DtDy * (fy1_0 - fy2_0);
+            const T y_u = DtDy * (fy1_2 - fy2_2);
+            const T y_v = DtDy * (fy1_1 - fy2_1);
+            const T y_w = DtDy * (fy1_3 - fy2_3);
+            const T y_E = DtDy * (fy1_4 - fy2_4);
+            const T z_r = DtDz * (fz1_0 - fz2_0);
+            const T z_u = DtDz * (fz1_3 - fz2_3);
+            const T z_v = DtDz * (fz1_2 - fz2_2);
+            const T z_w = DtDz * (fz1_1 - fz2_1);
+            const T z_E = DtDz * (fz1_4 - fz2_4);
+            const T nr = rcur.x + x_r + y_r + z_r;
+            const T nu = mcur.x + x_u + y_u + z_u;
+            const T nv = mcur.y + x_v + y_v + z_v;
+            const T nw = mcur.z + x_w + y_w + z_w;
+            const T nE = rcur.y + x_E + y_E + z_E;
+            if (owned) {
+                const long long n = static_cast<long long>(i) * g.zy + col;
+                V2 o;
+                o.x = nr;
+                o.y = nE;
+                RN2[n] = o;
+                Trip<T> m;
+                m.x = nu;
+                m.y = nv;
+                m.z = nw;
+                MN3[n] = m;
+                key = umax64(key, speed_key(static_cast<double>(signal_speed3<T>(nr, nu, nv, nw, nE, c.gmo, c.gamma))));
+            }
+            fx1_0 = fx2_0;
+            fx1_1 = fx2_1;
+            fx1_2 = fx2_2;
+            fx1_3 = fx2_3;
+            fx1_4 = fx2_4;
+        }
+        Pc = Pn;
+        rc = rn;
+        mc = mn;
+        rn = rnn;
+        mn = mnn;
+    }
+
+    const uint64_t wkey = wave_max_u64(key);
+    if (reduce_and_elect(red, wkey, gridDim.x)) {
+        const T dtn = finalize_dt<T>(red, c.C_min);
+        if constexpr (RESIDENT) {
+            const unsigned long long it = ctl->it;
+            if (traj_t) traj_t[it] = ctl->t;
+            if (traj_dt) traj_dt[it] = dtn;
+            const double t_new = ctl->t + static_cast<double>(dtn);  // hydro3d/main.cxx:874-875
+            ctl->dt = static_cast<double>(dtn);
+            ctl->t = t_new;
+            ctl->it = it + 1;
+            if (!isfinite(static_cast<double>(dtn))) {
+                ctl->nonfinite = 1u;
+                ctl->done = 1u;
+            }
+            if (!(t_new < ctl->max_t && it + 1 < ctl->max_it)) ctl->done = 1u;
+        } else {
+            *Dt_next = dtn;
+        }
+    }
+}
+
+// X ghost planes (hydro3d/main.cxx:374-405), periodic copy or reflective mirror.
+template <typename T>
+__global__ void ghost_x3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long long nb_x, long long zy, int bc_x,
+                                 const StepControl* ctl)
+{
+    using V2 = typename Vec2_3<T>::type;
+    if (ctl && ctl->done) return;
+    V2* R2 = reinterpret_cast<V2*>(rhoE);
+    Trip<T>* M3 = reinterpret_cast<Trip<T>*>(uvw);
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 2 * zy) return;
+    const int side = t >= zy;
+    const long long m = side ? t - zy : t;
+    const long long dst = side ? (nb_x + 1) * zy + m : m;
+    long long src;
+    if (bc_x == HYDRO_BC_PERIODIC) src = side ? zy + m : nb_x * zy + m;
+    else src = side ? nb_x * zy + m : zy + m;
+    R2[dst] = R2[src];
+    Trip<T> v = M3[src];
+    if (bc_x != HYDRO_BC_PERIODIC) v.x = -v.x;
+    M3[dst] = v;
+}
+
+// Periodic Y ghosts in every plane: row 0 <- row nb_y, row nb_y+1 <- row 1 (all k).
+template <typename T>
+__global__ void ghost_y3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long long nb_x, long long nb_y,
+                                 long long zs, const StepControl* ctl)
+{
+    using V2 = typename Vec2_3<T>::type;
+    if (ctl && ctl->done) return;
+    V2* R2 = reinterpret_cast<V2*>(rhoE);
+    Trip<T>* M3 = reinterpret_cast<Trip<T>*>(uvw);
+    const long long zy = (nb_y + 2) * zs;
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 2 * (nb_x + 2) * zs) return;
+    const long long k = t % zs;
+    const long long q = t / zs;
+    const int side = static_cast<int>(q & 1);
+    const long long i = q >> 1;
+    const long long dst = i * zy + (side ? nb_y + 1 : 0) * zs + k;
+    const long long src = i * zy + (side ? 1 : nb_y) * zs + k;
+    R2[dst] = R2[src];
+    M3[dst] = M3[src];
+}
+
+// Periodic Z ghosts in every (i, j) line: k = 0 <- k = nb_z, k = nb_z+1 <- k = 1.
+template <typename T>
+__global__ void ghost_z3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long long nb_x, long long nb_y,
+                                 long long nb_z, const StepControl* ctl)
+{
+    using V2 = typename Vec2_3<T>::type;
+    if (ctl && ctl->done) return;
+    V2* R2 = reinterpret_cast<V2*>(rhoE);
+    Trip<T>* M3 = reinterpret_cast<Trip<T>*>(uvw);
+    const long long zs = nb_z + 2;
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long lines = (nb_x + 2) * (nb_y + 2);
+    if (t >= 2 * lines) return;
+    const long long line = t >> 1;
+    const int side = static_cast<int>(t & 1);
+    const long long b = line * zs;
+    const long long dst = b + (side ? nb_z + 1 : 0);
+    const long long src = b + (side ? 1 : nb_z);
+    R2[dst] = R2[src];
+    M3[dst] = M3[src];
+}
+
+int g_rows_3d = 32;
+
+template <typename T>
+int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
+                  size_t nb_y, size_t nb_z, T DtDx, T DtDy, T DtDz, int bc_y, int bc_z, StepReduce* red,
+                  StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s)
+{
+    if (!rhoE || !uvw || !rhoE_next || !uvw_next || (!ctl && !Dt_next)) {
+        set_error("hydro 3D step: null pointer argument");
+        return HYDRO_ERR_ARG;
+    }
+    if (nb_x < 1 || nb_y < 1 || nb_z < 1 || nb_x > (1u << 30) || nb_y > (1u << 20) || nb_z > (1u << 20)) {
+        set_error("hydro 3D step: interior sizes out of range (%zu x %zu x %zu)", nb_x, nb_y, nb_z);
+        return HYDRO_ERR_ARG;
+    }
+    if ((bc_y != HYDRO_BC_PERIODIC && bc_y != HYDRO_BC_REFLECTIVE) ||
+        (bc_z != HYDRO_BC_PERIODIC && bc_z != HYDRO_BC_REFLECTIVE)) {
+        set_error("hydro 3D step: unknown boundary mode (%d, %d)", bc_y, bc_z);
+        return HYDRO_ERR_ARG;
+    }
+    Geom3 g;
+    g.nb_x = static_cast<int>(nb_x);
+    g.nb_y = static_cast<int>(nb_y);
+    g.nb_z = static_cast<int>(nb_z);
+    g.zs = static_cast<long long>(nb_z) + 2;
+    g.zy = (static_cast<long long>(nb_y) + 2) * g.zs;
+    g.rows = g_rows_3d;
+    g.nzw = static_cast<int>((nb_z + kOwnedZ - 1) / kOwnedZ);
+    g.nyw = static_cast<int>((nb_y + kTY - 1) / kTY);
+    const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
+    const long long nblocks = nstrips * g.nzw * g.nyw;
+    if (nblocks > 0x7fffffffLL) {
+        set_error("hydro 3D step: grid too large (%lld workgroups)", nblocks);
+        return HYDRO_ERR_ARG;
+    }
+    const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock3);
+    const bool wy = bc_y == HYDRO_BC_REFLECTIVE, wz = bc_z == HYDRO_BC_REFLECTIVE;
+#define HYDRO_L3(WY, WZ, RES)                                                                                 \
+    hipLaunchKernelGGL((fvm3d_kernel<T, WY, WZ, RES>), grid, block, 0, s, rhoE, uvw, rhoE_next, uvw_next, g, c, \
+                       DtDx, DtDy, DtDz, red, ctl, Dt_next, traj_t, traj_dt)
+    if (ctl) {
+        if (wy && wz) HYDRO_L3(true, true, true);
+        else if (wy) HYDRO_L3(true, false, true);
+        else if (wz) HYDRO_L3(false, true, true);
+        else HYDRO_L3(false, false, true);
+    } else {
+        if (wy && wz) HYDRO_L3(true, true, false);
+        else if (wy) HYDRO_L3(true, false, false);
+        else if (wz) HYDRO_L3(false, true, false);
+        else HYDRO_L3(false, false, false);
+    }
+#undef HYDRO_L3
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    return 0;
+}
+
+template <typename T>
+int launch_ghost3d(T* rhoE, T* uvw, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
+                   const StepControl* ctl, hipStream_t s)
+{
+    if (!rhoE || !uvw || nb_x < 1 || nb_y < 1 || nb_z < 1) {
+        set_error("hydro ghost fill 3D: bad arguments");
+        return HYDRO_ERR_ARG;
+    }
+    for (int b : { bc_x, bc_y, bc_z })
+        if (b != HYDRO_BC_PERIODIC && b != HYDRO_BC_REFLECTIVE) {
+            set_error("hydro ghost fill 3D: unknown boundary mode %d", b);
+            return HYDRO_ERR_ARG;
+        }
+    const long long zs = static_cast<long long>(nb_z) + 2;
+    const long long zy = (static_cast<long long>(nb_y) + 2) * zs;
+    hipLaunchKernelGGL((ghost_x3d_kernel<T>), dim3(static_cast<unsigned>((2 * zy + 255) / 256)), dim3(256), 0, s,
+                       rhoE, uvw, static_cast<long long>(nb_x), zy, bc_x, ctl);
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    if (bc_y == HYDRO_BC_PERIODIC) {
+        const long long n = 2 * (static_cast<long long>(nb_x) + 2) * zs;
+        hipLaunchKernelGGL((ghost_y3d_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                           rhoE, uvw, static_cast<long long>(nb_x), static_cast<long long>(nb_y), zs, ctl);
+        HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    }
+    if (bc_z == HYDRO_BC_PERIODIC) {
+        const long long n = 2 * (static_cast<long long>(nb_x) + 2) * (static_cast<long long>(nb_y) + 2);
+        hipLaunchKernelGGL((ghost_z3d_kernel<T>), dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                           rhoE, uvw, static_cast<long long>(nb_x), static_cast<long long>(nb_y),
+                           static_cast<long long>(nb_z), ctl);
+        HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    }
+    return 0;
+}
+
+
+StepReduce* plain_reduce_state3(int which)
+{
+    static StepReduce* addr[64] = {};
+    const int dev = current_device();
+    if (dev < 0 || dev >= 64) return nullptr;
+    if (!addr[dev]) {
+        void* p = nullptr;
+        HYDRO_HIP_OR(hipGetSymbolAddress(&p, HIP_SYMBOL(g_red3d)), nullptr);
+        addr[dev] = static_cast<StepReduce*>(p);
+    }
+    return addr[dev] + which;
+}
+
+struct EventPair3 {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+EventPair3* timing_events3()
+{
+    static thread_local EventPair3 ev[64];
+    const int dev = current_device();
+    if (dev < 0 || dev >= 64) return nullptr;
+    EventPair3& e = ev[dev];
+    if (!e.a) {
+        HYDRO_HIP_OR(hipEventCreate(&e.a), nullptr);
+        HYDRO_HIP_OR(hipEventCreate(&e.b), nullptr);
+    }
+    return &e;
+}
+
+template <typename T>
+double timed_launcher3d(const T* d_rhoE, const T* d_uvw, T* d_rhoE_next, T* d_uvw_next, T* Dt_next, T C_min, T gamma,
+                        T gamma_minus_one, T divgamma, T K, size_t NB_X, size_t NB_Y, size_t NB_Z, T DtDx, T DtDy,
+                        T DtDz, void* stream)
+{
+    clear_error();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StepReduce* red = plain_reduce_state3(sizeof(T) == 8 ? 0 : 1);
+    EventPair3* ev = timing_events3();
+    if (!red || !ev) return HYDRO_ERR_HIP;
+    const Consts3<T> c{ C_min, gamma, gamma_minus_one, divgamma, K };
+    HYDRO_HIP_OR(hipEventRecord(ev->a, s), HYDRO_ERR_HIP);
+    int rc = launch_step3d<T>(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, c, NB_X, NB_Y, NB_Z, DtDx, DtDy, DtDz,
+                              HYDRO_BC_REFLECTIVE, HYDRO_BC_REFLECTIVE, red, nullptr, Dt_next, nullptr, nullptr, s);
+    if (rc) return rc;
+    HYDRO_HIP_OR(hipEventRecord(ev->b, s), HYDRO_ERR_HIP);
+    HYDRO_HIP_OR(hipEventSynchronize(ev->b), HYDRO_ERR_HIP);
+    float ms = 0.f;
+    HYDRO_HIP_OR(hipEventElapsedTime(&ms, ev->a, ev->b), HYDRO_ERR_HIP);
+    return static_cast<double>(ms) * 1e3;
+}
+
+}  // namespace
+
+int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
+                    size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
+                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s)
+{
+    if (precision == 64) {
+        const Consts3<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
+        return launch_step3d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
+                                     static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c, nb_x, nb_y,
+                                     nb_z, 0.0, 0.0, 0.0, bc_y, bc_z, red, ctl, nullptr, traj_t,
+                                     static_cast<double*>(traj_dt), s);
+    }
+    const Consts3<float> c{ static_cast<float>(consts[0]), static_cast<float>(consts[1]),
+                            static_cast<float>(consts[2]), static_cast<float>(consts[3]),
+                            static_cast<float>(consts[4]) };
+    return launch_step3d<float>(static_cast<const float*>(rhoE), static_cast<const float*>(mom),
+                                static_cast<float*>(rhoE_next), static_cast<float*>(mom_next), c, nb_x, nb_y, nb_z,
+                                0.f, 0.f, 0.f, bc_y, bc_z, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s);
+}
+
+int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
+                int precision, const StepControl* ctl, hipStream_t s)
+{
+    if (precision == 64)
+        return launch_ghost3d<double>(static_cast<double*>(rhoE), static_cast<double*>(mom), nb_x, nb_y, nb_z, bc_x,
+                                      bc_y, bc_z, ctl, s);
+    return launch_ghost3d<float>(static_cast<float*>(rhoE), static_cast<float*>(mom), nb_x, nb_y, nb_z, bc_x, bc_y,
+                                 bc_z, ctl, s);
+}
+
+void set_tile_3d(int rows)
+{
+    if (rows >= 2 && rows <= 4096) g_rows_3d = rows;
+}
+
+}  // namespace hydro
+
+using namespace hydro;
+
+extern "C" double launcher3d(const double* d_rhoE, const double* d_uvw, double* d_rhoE_next, double* d_uvw_next,
+                             double* Dt_next, double C_min, double gamma, double gamma_minus_one, double divgamma,
+                             double K, size_t NB_X, size_t NB_Y, size_t NB_Z, double DtDx, double DtDy, double DtDz,
+                             void* stream)
+{
+    return timed_launcher3d<double>(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, Dt_next, C_min, gamma, gamma_minus_one,
+                                    divgamma, K, NB_X, NB_Y, NB_Z, DtDx, DtDy, DtDz, stream);
+}
+
+extern "C" double launcher3d_f32(const float* d_rhoE, const float* d_uvw, float* d_rhoE_next, float* d_uvw_next,
+                                 float* Dt_next, float C_min, float gamma, float gamma_minus_one, float divgamma,
+                                 float K, size_t NB_X, size_t NB_Y, size_t NB_Z, float DtDx, float DtDy, float DtDz,
+                                 void* stream)
+{
+    return timed_launcher3d<float>(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, Dt_next, C_min, gamma, gamma_minus_one,
+                                   divgamma, K, NB_X, NB_Y, NB_Z, DtDx, DtDy, DtDz, stream);
+}
+
+extern "C" int hydro_step3d(const double* d_rhoE, const double* d_uvw, double* d_rhoE_next, double* d_uvw_next,
+                            double* Dt_next, double C_min, double gamma, double gamma_minus_one, double divgamma,
+                            double K, size_t NB_X, size_t NB_Y, size_t NB_Z, double DtDx, double DtDy, double DtDz,
+                            int bc_y, int bc_z, void* stream)
+{
+    clear_error();
+    StepReduce* red = plain_reduce_state3(0);
+    if (!red) return HYDRO_ERR_HIP;
+    const Consts3<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };
+    return launch_step3d<double>(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, c, NB_X, NB_Y, NB_Z, DtDx, DtDy, DtDz, bc_y,
+                                 bc_z, red, nullptr, Dt_next, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hydro_ghost_fill3d(double* d_rhoE, double* d_uvw, size_t NB_X, size_t NB_Y, size_t NB_Z, int bc_x,
+                                  int bc_y, int bc_z, void* stream)
+{
+    clear_error();
+    return launch_ghost3d<double>(d_rhoE, d_uvw, NB_X, NB_Y, NB_Z, bc_x, bc_y, bc_z, nullptr,
+                                  static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hydro_ghost_fill3d_f32(float* d_rhoE, float* d_uvw, size_t NB_X, size_t NB_Y, size_t NB_Z, int bc_x,
+                                      int bc_y, int bc_z, void* stream)
+{
+    clear_error();
+    return launch_ghost3d<float>(d_rhoE, d_uvw, NB_X, NB_Y, NB_Z, bc_x, bc_y, bc_z, nullptr,
+                                 static_cast<hipStream_t>(stream));
+}

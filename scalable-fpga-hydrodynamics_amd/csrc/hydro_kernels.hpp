@@ -1,0 +1,27 @@
+// Internal launch interface between the HIP translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "hydro_device.hpp"
+
+namespace hydro {
+
+// consts = { C_min, gamma, gamma_minus_one, divgamma, K } in the state precision.
+int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
+                    int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
+                    void* traj_dt, hipStream_t s);
+int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
+                    size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
+                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s);
+
+int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,
+                const StepControl* ctl, hipStream_t s);
+int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
+                int precision, const StepControl* ctl, hipStream_t s);
+
+void set_rows_2d(int rows);
+void set_tile_3d(int rows);
+
+}  // namespace hydro

@@ -1,0 +1,344 @@
+// Device-resident step loop and diagnostics.
+//
+// hydro_run moves the host loop of hydro2d/main.cxx:686-815 (hydro3d/main.cxx:742-876)
+// onto the device: Dt, t and the iteration count live in a StepControl block, each
+// step's kernels read Dt from it (DtDx = Dt / DX computed on the device, the same IEEE
+// division the host performs) and the workgroup that finalises the Dt reduction
+// records (t, Dt), advances t += Dt and sets `done` when the loop condition fails.  The
+// host can therefore enqueue many steps back to back (or capture them in a graph)
+// without reading Dt between steps.
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/hydro_hip.h"
+#include "hydro_device.hpp"
+#include "hydro_kernels.hpp"
+#include "hydro_runtime.hpp"
+
+using namespace hydro;
+
+struct hydro_run {
+    int dim = 0;
+    int precision = 64;
+    size_t nb[3] = { 0, 0, 0 };
+    int bc[3] = { HYDRO_BC_PERIODIC, HYDRO_BC_REFLECTIVE, HYDRO_BC_REFLECTIVE };
+    double consts[5] = {};  // C_min, gamma, gamma - 1, 1 / (gamma - 1), K
+    StepControl* d_ctl = nullptr;
+    double* d_traj_t = nullptr;
+    void* d_traj_dt = nullptr;
+    size_t max_it = 0;
+    size_t enqueued = 0;
+};
+
+namespace {
+
+// gamma, K and the derived constants of hydro2d/main.cxx:601-606 in the state precision.
+void physics_constants(int precision, double C_min, double out[5])
+{
+    if (precision == 64) {
+        const double gamma = 1.4, K = 1.1;
+        const double gmo = gamma - 1.0;
+        out[0] = C_min;
+        out[1] = gamma;
+        out[2] = gmo;
+        out[3] = 1.0 / gmo;
+        out[4] = K;
+    } else {
+        const float gamma = 1.4f, K = 1.1f;
+        const float gmo = gamma - 1.0f;
+        out[0] = static_cast<float>(C_min);
+        out[1] = gamma;
+        out[2] = gmo;
+        out[3] = 1.0f / gmo;
+        out[4] = K;
+    }
+}
+
+}  // namespace
+
+extern "C" hydro_run* hydro_run_create(int dim, const size_t* nb, const double* D, double C_min, double dt0,
+                                       double max_t, size_t max_it, int precision, const int* bc)
+{
+    clear_error();
+    if ((dim != 2 && dim != 3) || !nb || !D || (precision != 64 && precision != 32)) {
+        set_error("hydro_run_create: bad dim/precision/pointer arguments");
+        return nullptr;
+    }
+    for (int d = 0; d < dim; ++d) {
+        if (nb[d] < 1) {
+            set_error("hydro_run_create: nb[%d] must be >= 1", d);
+            return nullptr;
+        }
+        if (bc && bc[d] != HYDRO_BC_PERIODIC && bc[d] != HYDRO_BC_REFLECTIVE) {
+            set_error("hydro_run_create: unknown boundary mode %d on axis %d", bc[d], d);
+            return nullptr;
+        }
+    }
+    hydro_run* r = new (std::nothrow) hydro_run();
+    if (!r) {
+        set_error("hydro_run_create: out of host memory");
+        return nullptr;
+    }
+    r->dim = dim;
+    r->precision = precision;
+    for (int d = 0; d < dim; ++d) {
+        r->nb[d] = nb[d];
+        if (bc) r->bc[d] = bc[d];
+    }
+    r->max_it = max_it;
+    physics_constants(precision, C_min, r->consts);
+
+    StepControl h;
+    memset(&h, 0, sizeof(h));
+    h.dt = precision == 64 ? dt0 : static_cast<double>(static_cast<float>(dt0));
+    h.t = 0.0;
+    h.max_t = max_t;
+    h.it = 0;
+    h.max_it = max_it;
+    h.C_min = r->consts[0];
+    for (int d = 0; d < dim; ++d) h.D[d] = precision == 64 ? D[d] : static_cast<double>(static_cast<float>(D[d]));
+    h.done = (0.0 < max_t && 0 < max_it) ? 0u : 1u;  // hydro2d/main.cxx:689
+    const size_t traj_bytes_dt = (max_it ? max_it : 1) * (precision == 64 ? 8 : 4);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->d_ctl), sizeof(StepControl));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&r->d_traj_t), (max_it ? max_it : 1) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&r->d_traj_dt, traj_bytes_dt);
+    if (e == hipSuccess) e = hipMemcpy(r->d_ctl, &h, sizeof(h), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error("hydro_run_create: %s", hipGetErrorString(e));
+        hydro_run_destroy(r);
+        return nullptr;
+    }
+    return r;
+}
+
+extern "C" int hydro_run_advance(hydro_run* r, void* rhoE_a, void* mom_a, void* rhoE_b, void* mom_b, size_t n_steps,
+                                 void* stream)
+{
+    clear_error();
+    if (!r || !rhoE_a || !mom_a || !rhoE_b || !mom_b) {
+        set_error("hydro_run_advance: null argument");
+        return HYDRO_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (size_t n = 0; n < n_steps; ++n) {
+        const bool in_a = (r->enqueued % 2) == 0;
+        void* cr = in_a ? rhoE_a : rhoE_b;
+        void* cm = in_a ? mom_a : mom_b;
+        void* nr = in_a ? rhoE_b : rhoE_a;
+        void* nm = in_a ? mom_b : mom_a;
+        int rc;
+        if (r->dim == 2) {
+            rc = ghost2d_any(cr, cm, r->nb[0], r->nb[1], r->bc[0], r->bc[1], r->precision, r->d_ctl, s);
+            if (!rc)
+                rc = step2d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->bc[1], r->precision, r->consts, r->d_ctl,
+                                     &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s);
+        } else {
+            rc = ghost3d_any(cr, cm, r->nb[0], r->nb[1], r->nb[2], r->bc[0], r->bc[1], r->bc[2], r->precision,
+                             r->d_ctl, s);
+            if (!rc)
+                rc = step3d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->nb[2], r->bc[1], r->bc[2], r->precision,
+                                     r->consts, r->d_ctl, &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s);
+        }
+        if (rc) return rc;
+        ++r->enqueued;
+    }
+    return 0;
+}
+
+extern "C" int hydro_run_status(hydro_run* r, size_t* it, double* t, double* dt, int* done, int* current,
+                                void* stream)
+{
+    clear_error();
+    if (!r) {
+        set_error("hydro_run_status: null run");
+        return HYDRO_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StepControl h;
+    HYDRO_HIP_OR(hipMemcpyAsync(&h, r->d_ctl, sizeof(h), hipMemcpyDeviceToHost, s), HYDRO_ERR_HIP);
+    HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
+    if (it) *it = static_cast<size_t>(h.it);
+    if (t) *t = h.t;
+    if (dt) *dt = h.dt;
+    if (done) *done = h.done ? 1 : 0;
+    if (current) *current = static_cast<int>(h.it % 2);
+    if (h.nonfinite) {
+        set_error("hydro run: Dt became non-finite at iteration %llu (NaN or zero signal speed)",
+                  static_cast<unsigned long long>(h.it ? h.it - 1 : 0));
+        return HYDRO_ERR_NONFINITE;
+    }
+    return 0;
+}
+
+extern "C" int hydro_run_trajectory(hydro_run* r, size_t from, size_t count, double* t_out, double* dt_out,
+                                    void* stream)
+{
+    clear_error();
+    if (!r || from + count > r->max_it) {
+        set_error("hydro_run_trajectory: range [%zu, %zu) outside the %zu recorded iterations", from, from + count,
+                  r ? r->max_it : 0);
+        return HYDRO_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (count == 0) return 0;
+    if (t_out)
+        HYDRO_HIP_OR(hipMemcpyAsync(t_out, r->d_traj_t + from, count * sizeof(double), hipMemcpyDeviceToHost, s),
+                     HYDRO_ERR_HIP);
+    if (dt_out) {
+        if (r->precision == 64) {
+            HYDRO_HIP_OR(hipMemcpyAsync(dt_out, static_cast<double*>(r->d_traj_dt) + from, count * sizeof(double),
+                                        hipMemcpyDeviceToHost, s),
+                         HYDRO_ERR_HIP);
+        } else {
+            float* tmp = static_cast<float*>(malloc(count * sizeof(float)));
+            if (!tmp) {
+                set_error("hydro_run_trajectory: out of host memory");
+                return HYDRO_ERR_NOMEM;
+            }
+            hipError_t e = hipMemcpyAsync(tmp, static_cast<float*>(r->d_traj_dt) + from, count * sizeof(float),
+                                          hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                free(tmp);
+                set_error("hydro_run_trajectory: %s", hipGetErrorString(e));
+                return HYDRO_ERR_HIP;
+            }
+            for (size_t i = 0; i < count; ++i) dt_out[i] = tmp[i];
+            free(tmp);
+        }
+    }
+    HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
+    return 0;
+}
+
+extern "C" void hydro_run_destroy(hydro_run* r)
+{
+    if (!r) return;
+    if (r->d_ctl) (void)hipFree(r->d_ctl);
+    if (r->d_traj_t) (void)hipFree(r->d_traj_t);
+    if (r->d_traj_dt) (void)hipFree(r->d_traj_dt);
+    delete r;
+}
+
+// ---------------------------------------------------------------------------------
+// Total mass: sum of interior rho with error-free two-sum accumulation (each lane keeps
+// a (sum, error) pair; pairs are merged in a tree), then one atomic-free finish by the
+// last workgroup.  The reference sums on the host in long double with Neumaier
+// compensation (hydro2d/main.cxx:237-265); both are accurate far below the printed
+// 8 significant digits.
+// ---------------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e)
+{
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double bhi, double blo)
+{
+    double s, e;
+    two_sum(hi, bhi, s, e);
+    e = e + lo + blo;
+    two_sum(s, e, hi, lo);
+}
+
+// Interior cell (i, j, k) in [0, nb_x) x [0, nb_y) x [0, nb_z) lives at element
+// base + i*sx + j*sy + k*sz of the cell array (2D: nb_z = 1, sz = 0).
+template <typename T>
+__global__ void __launch_bounds__(256) mass_kernel(const T* __restrict__ rhoE, long long nb_x, long long nb_y,
+                                                   long long nb_z, long long base, long long sx, long long sy,
+                                                   long long sz, double* partial)
+{
+    const long long ncell = nb_x * nb_y * nb_z;
+    double hi = 0.0, lo = 0.0;
+    for (long long m = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; m < ncell;
+         m += static_cast<long long>(gridDim.x) * blockDim.x) {
+        const long long k = m % nb_z;
+        const long long q = m / nb_z;
+        const long long j = q % nb_y;
+        const long long i = q / nb_y;
+        const long long n = base + i * sx + j * sy + k * sz;
+        dd_add(hi, lo, static_cast<double>(rhoE[2 * n]), 0.0);
+    }
+    __shared__ double sh[256], sl[256];
+    sh[threadIdx.x] = hi;
+    sl[threadIdx.x] = lo;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (static_cast<int>(threadIdx.x) < off) {
+            double h = sh[threadIdx.x], l = sl[threadIdx.x];
+            dd_add(h, l, sh[threadIdx.x + off], sl[threadIdx.x + off]);
+            sh[threadIdx.x] = h;
+            sl[threadIdx.x] = l;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = sh[0];
+        partial[2 * blockIdx.x + 1] = sl[0];
+    }
+}
+
+__global__ void mass_finish_kernel(const double* partial, int n, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double hi = 0.0, lo = 0.0;
+    for (int b = 0; b < n; ++b) dd_add(hi, lo, partial[2 * b], partial[2 * b + 1]);
+    out[0] = hi + lo;
+}
+
+}  // namespace
+
+extern "C" int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision,
+                                double* mass, void* stream)
+{
+    clear_error();
+    if (!d_rhoE || !mass || NB_X < 1 || NB_Y < 1 || (precision != 64 && precision != 32)) {
+        set_error("hydro_total_mass: bad arguments");
+        return HYDRO_ERR_ARG;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int nblk = 1024;
+    double* d_partial = nullptr;
+    HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&d_partial), (2 * nblk + 1) * sizeof(double)), HYDRO_ERR_HIP);
+    long long nz, sx, sy, sz;
+    if (NB_Z) {
+        nz = static_cast<long long>(NB_Z);
+        sz = 1;
+        sy = nz + 2;
+        sx = (static_cast<long long>(NB_Y) + 2) * sy;
+    } else {
+        nz = 1;
+        sz = 0;
+        sy = 1;
+        sx = static_cast<long long>(NB_Y) + 2;
+    }
+    const long long base = sx + sy + sz;
+    hipError_t e = hipSuccess;
+    if (precision == 64)
+        hipLaunchKernelGGL(mass_kernel<double>, dim3(nblk), dim3(256), 0, s, static_cast<const double*>(d_rhoE),
+                           static_cast<long long>(NB_X), static_cast<long long>(NB_Y), nz, base, sx, sy, sz,
+                           d_partial);
+    else
+        hipLaunchKernelGGL(mass_kernel<float>, dim3(nblk), dim3(256), 0, s, static_cast<const float*>(d_rhoE),
+                           static_cast<long long>(NB_X), static_cast<long long>(NB_Y), nz, base, sx, sy, sz,
+                           d_partial);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mass_finish_kernel, dim3(1), dim3(64), 0, s, d_partial, nblk, d_partial + 2 * nblk);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(mass, d_partial + 2 * nblk, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_partial);
+    if (e != hipSuccess) {
+        set_error("hydro_total_mass: %s", hipGetErrorString(e));
+        return HYDRO_ERR_HIP;
+    }
+    return 0;
+}
