@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the first-order FV Euler step on one MI355X.
+
+A step is one pass of the hot path over the whole grid: the device ghost fill plus the
+fused flux/update/Dt kernel, with Dt, t and the iteration count resident on the device
+(the step loop of hydro2d/main.cxx:689-815).  Default workload: the plain 2D blast on a
+40000 x 20000 grid in FP64 (the reference's built-in IC scaled to the grid, periodic X,
+reflective Y), 100 timed steps after a short warmup, state resident in HBM.
+
+Prints ONE JSON line: GCUPS (1e9 cell updates per second) over the timed steps, the HBM
+roofline of the step kernel (algorithmic 64 B per 2D cell update, 80 B in 3D, over the
+kernel's average HIP-event duration) and the CPU oracle timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+WORKLOADS = {
+    # name: (dim, nb, C, ic_mode, radius_frac, p_in, p_out, bytes per cell update)
+    "blast2d_40000x20000_fp64": (2, (40000, 20000), 0.5, 0, 0.0, 10.0, 0.1, 64),
+    "blast3d_512_fp64": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80),
+}
+ALIASES = {"2d": "blast2d_40000x20000_fp64", "3d": "blast3d_512_fp64"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(dim: int, budget_s: float = 12.0) -> dict:
+    """The CPU oracle (a restatement of the reference step, OpenMP over X rows) timed on
+    this host on a bounded sample of the same blast: 4000 x 2000 in 2D, 128^3 in 3D."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    if dim == 2:
+        nb = (4000, 2000)
+        rhoE, mom, dt0 = O.ic2d(*nb)
+        run = lambda n: O.run2d(rhoE, mom, dt0, 2.0, n, nthreads=threads)  # noqa: E731
+    else:
+        nb = (128, 128, 128)
+        rhoE, mom, dt0 = O.ic3d(*nb, ic_mode=1, radius_frac=0.1)
+        run = lambda n: O.run3d(rhoE, mom, dt0, 2.0, n, nthreads=threads)  # noqa: E731
+    cells = int(np.prod(nb))
+    t0 = time.perf_counter()
+    run(2)
+    per_step = (time.perf_counter() - t0) / 2
+    steps = int(max(2, min(200, budget_s / max(per_step, 1e-6))))
+    t0 = time.perf_counter()
+    res = run(steps)
+    el = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {
+        "value": cells * res["it"] / el / 1e9,
+        "unit": "GCUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{'x'.join(map(str, nb))} blast, {res['it']} steps, FP64, {threads} OpenMP threads"
+                  f" ({el:.1f} s){' on ' + cpu_model if cpu_model else ''}",
+    }
+
+
+def load_traffic(workload: str):
+    p = ROOT / "profiles" / "traffic.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get(workload, {}).get("bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="2d")
+    ap.add_argument("--rows", type=int, default=0, help="override the kernel's X rows per strip")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X and the "
+              "solver keeps one resident domain per GPU; run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    name = ALIASES.get(args.workload, args.workload)
+    if name not in WORKLOADS:
+        log(f"unknown workload {args.workload}; choose from {sorted(WORKLOADS)}")
+        return 2
+    dim, nb, C_cfl, ic_mode, rfrac, p_in, p_out, bpc = WORKLOADS[name]
+
+    import numpy as np
+    import torch
+
+    hydro = importlib.import_module("scalable-fpga-hydrodynamics_amd")
+    if not torch.cuda.is_available():
+        log("no GPU visible")
+        return 1
+    hydro.lib()
+    if args.rows:
+        hydro.tune_set("rows2d" if dim == 2 else "rows3d", args.rows)
+
+    k = hydro.constants(nb, C_cfl)
+    shape = tuple(n + 2 for n in nb)
+    mom_c = 2 if dim == 2 else 3
+    log(f"[bench] {name}: allocating {4 * np.prod(shape) * (2 + mom_c) / 2 * 8 / 1e9:.1f} GB")
+    a = [torch.empty(shape + (2,), dtype=torch.float64, device="cuda"),
+         torch.empty(shape + (mom_c,), dtype=torch.float64, device="cuda")]
+    b = [torch.empty_like(a[0]), torch.empty_like(a[1])]
+    for t in a + b:
+        t.zero_()
+    dt0 = hydro.ic(a[0], a[1], k["C_min"], ic_mode=ic_mode, radius_frac=rfrac, p_in=p_in, p_out=p_out)
+    mass0 = hydro.total_mass(a[0])
+    log(f"[bench] IC done: dt0={dt0:.3e} mass0={mass0:.7e}")
+
+    W, K = args.warmup, args.steps
+    run = hydro.Run(nb, k["D"], k["C_min"], dt0, 2.0, W + K)
+    run.advance(a, b, W)
+    st = run.status()
+    if st["rc"] != 0:
+        log(f"[bench] warmup failed: {st}")
+        return 1
+    log(f"[bench] warmup {W} steps done, t={st['t']:.4e}")
+
+    run.set_timing(True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    ev0.record()
+    run.advance(a, b, K)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    gpu_s = ev0.elapsed_time(ev1) / 1e3
+    kernel_ms, launches = run.kernel_time()
+    st = run.status()
+    if st["rc"] != 0 or st["it"] != W + K:
+        log(f"[bench] timed run failed: {st}")
+        return 1
+    cur = a if st["current"] == 0 else b
+    mass1 = hydro.total_mass(cur[0])
+    cells = int(np.prod(nb))
+    elapsed = max(wall, gpu_s)
+    gcups = cells * K / elapsed / 1e9
+    avg_kernel_s = kernel_ms / 1e3 / max(launches, 1)
+    achieved = bpc * cells / avg_kernel_s / 1e9
+    log(f"[bench] {K} steps: wall {wall * 1e3:.1f} ms, events {gpu_s * 1e3:.1f} ms, kernel avg "
+        f"{avg_kernel_s * 1e3:.3f} ms x {launches}, {gcups:.2f} GCUPS, {achieved:.0f} GB/s")
+
+    if args.calibrate_copy:
+        # A plain device-to-device copy of exactly 1 GiB (read 1 GiB, write 1 GiB): what
+        # the FETCH_SIZE / WRITE_SIZE counters read for it scales the kernel's counts.
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU oracle on this host ...")
+        cpu = cpu_baseline(dim)
+        log(f"[bench] cpu: {cpu}")
+
+    line = {
+        "metric": "cell-updates/s (GCUPS) + % HBM roofline",
+        "value": round(gcups, 4),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(elapsed / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (built-in blast IC generated on the device)",
+        "config": {"workload": name, "grid": list(nb), "cells": cells, "boundaries": "periodic X, reflective Y/Z",
+                   "steps_timed": K, "kernel": "fvm2d" if dim == 2 else "fvm3d",
+                   "rows_per_strip": hydro.tune_get("rows2d" if dim == 2 else "rows3d")},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(name)},
+        "cpu_baseline": cpu,
+        "hbm_frac_whole_step": round(bpc * cells * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": round(avg_kernel_s * 1e3, 4),
+        "mass_change_per_cell": abs(mass1 - mass0) / cells,
+        "t_final": st["t"],
+    }
+    print(json.dumps(line), flush=True)
+    run.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -129,7 +129,21 @@ int hydro_run_status(hydro_run* run, size_t* it, double* t, double* dt, int* don
 /* Copy recorded (t, Dt) pairs [from, from+count) to host arrays (either may be NULL). */
 int hydro_run_trajectory(hydro_run* run, size_t from, size_t count, double* t_out, double* dt_out, void* stream);
 
+/* Record HIP events around every step-kernel launch of later hydro_run_advance calls
+ * (enable != 0), for measuring the kernel alone. */
+int hydro_run_set_timing(hydro_run* run, int enable);
+
+/* Wait for the recorded launches and return the sum of their kernel times in ms and
+ * their count; clears the record. */
+int hydro_run_kernel_time(hydro_run* run, double* total_ms, size_t* launches);
+
 void hydro_run_destroy(hydro_run* run);
+
+/* Tuning knobs of the launch geometry (no effect on results): "rows2d" = X rows per
+ * wavefront strip of the 2D kernel, "rows3d" = X rows per workgroup strip of the 3D
+ * kernel.  Returns 0, or HYDRO_ERR_ARG for an unknown key or out-of-range value. */
+int hydro_tune_set(const char* key, int value);
+int hydro_tune_get(const char* key);
 
 /* ---------------------------------------------------------------------------------
  * Initial condition.

@@ -74,6 +74,10 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_run_trajectory.argtypes = [vp, sz, sz, vp, vp, vp]
     L.hydro_run_destroy.argtypes = [vp]
     L.hydro_run_destroy.restype = None
+    L.hydro_run_set_timing.argtypes = [vp, i32]
+    L.hydro_run_kernel_time.argtypes = [vp, vp, vp]
+    L.hydro_tune_set.argtypes = [C.c_char_p, i32]
+    L.hydro_tune_get.argtypes = [C.c_char_p]
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
@@ -253,6 +257,15 @@ class Run:
                                           td.ctypes.data_as(C.c_void_p), _stream(stream)), "hydro_run_trajectory")
         return tt[:n], td[:n]
 
+    def set_timing(self, enable: bool = True) -> None:
+        _check(lib().hydro_run_set_timing(self._h, 1 if enable else 0), "hydro_run_set_timing")
+
+    def kernel_time(self):
+        """(total ms, launches) of the step kernels recorded since set_timing / the last call."""
+        ms, n = C.c_double(), C.c_size_t()
+        _check(lib().hydro_run_kernel_time(self._h, C.byref(ms), C.byref(n)), "hydro_run_kernel_time")
+        return ms.value, n.value
+
     def close(self):
         if getattr(self, "_h", None):
             lib().hydro_run_destroy(self._h)
@@ -263,6 +276,15 @@ class Run:
             self.close()
         except Exception:
             pass
+
+
+def tune_set(key: str, value: int) -> None:
+    """Launch-geometry knobs ("rows2d", "rows3d"); results do not depend on them."""
+    _check(lib().hydro_tune_set(key.encode(), int(value)), f"hydro_tune_set({key})")
+
+
+def tune_get(key: str) -> int:
+    return lib().hydro_tune_get(key.encode())
 
 
 def library_path() -> str:

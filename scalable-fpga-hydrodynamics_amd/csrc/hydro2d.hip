@@ -109,7 +109,21 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
     const int win = wb * kWavesPerBlock + wv;
     const int j = win * kOwned + lane;
     const bool owned = (win < g.nwin) && lane >= 1 && lane <= kOwned && j <= g.nb_y;
-    const int jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;  // clamped so every load is in bounds
+    // Column actually loaded by this lane.  With reflective walls the two ghost columns
+    // load their interior neighbour and present its mirror (v negated) to the Y face,
+    // which is the reference's wall state (kernel.cxx:214-229, 234-250): lane j = 0
+    // evaluates flux(mirror(c1), c1) for column 1, and lane j = nb_y + 1 hands
+    // mirror(c_nb_y) to column nb_y.  Negation is exact, so prims(mirror) == mirror(prims).
+    // Periodic Y reads the ghost columns filled by the ghost fill.  Lanes past nb_y + 1
+    // are clamped in bounds and never owned.
+    int jl;
+    unsigned int ymask = 0u;
+    if constexpr (WALL_Y) {
+        jl = j < 1 ? 1 : (j > g.nb_y ? g.nb_y : j);
+        ymask = (j == 0 || j == g.nb_y + 1) ? kSignMask<T> : 0u;
+    } else {
+        jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
+    }
     const int i0 = 1 + strip * g.rows;
     const int i1 = min(i0 + g.rows - 1, g.nb_x);
 
@@ -128,16 +142,20 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
         T fx1_0, fx1_1, fx1_2, fx1_3;
         face_flux2<T>(Pm, Pc, c.K, c.divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
 
+        // The loop body has no branch except the final store, so the compiler can
+        // overlap the independent work of consecutive rows.
+#pragma unroll 2
         for (int i = i0; i <= i1; ++i) {
             // Prefetch row i+2 (clamped to the last row this strip reads).
             V2 rnn, mnn;
             const int ipf = min(i + 2, i1 + 1);
             load_cell<T>(R2, M2, static_cast<long long>(ipf) * g.ys + jl, rnn, mnn);
 
-            // Y faces of row i, in the (rho, v, u, p) frame (kernel.cxx:213-251).
+            // Y face (j, j+1) of row i in the (rho, v, u, p) frame (kernel.cxx:213-251),
+            // evaluated by lane j and handed to lane j+1.
             Prim2<T> Sy, Ny;
             Sy.r = Pc.r;
-            Sy.un = Pc.ut;
+            Sy.un = xor_sign(Pc.ut, ymask);
             Sy.ut = Pc.un;
             Sy.p = Pc.p;
             Sy.a = Pc.a;
@@ -148,21 +166,6 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
             Ny.a = from_next_lane(Sy.a);
             T fyR0, fyR1, fyR2, fyR3;
             face_flux2<T>(Sy, Ny, c.K, c.divgamma, fyR0, fyR1, fyR2, fyR3);
-            if constexpr (WALL_Y) {
-                // Reflective walls: the outside state is the cell itself with v negated.
-                // Lane of column nb_y evaluates flux(c, mirror(c)) (kernel.cxx:233-251);
-                // the lane of ghost column 0 evaluates flux(mirror(c1), c1), which
-                // column 1 receives as its fy1 (kernel.cxx:213-230).
-                const bool bottom = (j == g.nb_y);
-                const bool top = (j == 0);
-                if (bottom || top) {
-                    Prim2<T> L = bottom ? Sy : Ny;
-                    Prim2<T> R = L;
-                    if (bottom) R.un = -Sy.un;
-                    else L.un = -Ny.un;
-                    face_flux2<T>(L, R, c.K, c.divgamma, fyR0, fyR1, fyR2, fyR3);
-                }
-            }
             const T fyL0 = from_prev_lane(fyR0);
             const T fyL1 = from_prev_lane(fyR1);
             const T fyL2 = from_prev_lane(fyR2);
@@ -189,6 +192,8 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
             const T nu = mc.x + x_u + y_u;
             const T nv = mc.y + x_v + y_v;
             const T nE = rc.y + x_E + y_E;
+            const uint64_t k_new = speed_key(static_cast<double>(signal_speed2<T>(nr, nu, nv, nE, c.gmo, c.gamma)));
+            key = owned ? umax64(key, k_new) : key;
             if (owned) {
                 const long long n = static_cast<long long>(i) * g.ys + j;
                 V2 o1, o2;
@@ -198,7 +203,6 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
                 o2.y = nv;
                 RN2[n] = o1;
                 MN2[n] = o2;
-                key = umax64(key, speed_key(static_cast<double>(signal_speed2<T>(nr, nu, nv, nE, c.gmo, c.gamma))));
             }
 
             // Advance the march.
@@ -214,26 +218,7 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
         }
     }
 
-    const uint64_t wkey = wave_max_u64(key);
-    if (reduce_and_elect(red, wkey, gridDim.x)) {
-        const T dtn = finalize_dt<T>(red, c.C_min);
-        if constexpr (RESIDENT) {
-            const unsigned long long it = ctl->it;
-            if (traj_t) traj_t[it] = ctl->t;
-            if (traj_dt) traj_dt[it] = dtn;
-            const double t_new = ctl->t + static_cast<double>(dtn);  // hydro2d/main.cxx:813-814
-            ctl->dt = static_cast<double>(dtn);
-            ctl->t = t_new;
-            ctl->it = it + 1;
-            if (!isfinite(static_cast<double>(dtn))) {
-                ctl->nonfinite = 1u;
-                ctl->done = 1u;
-            }
-            if (!(t_new < ctl->max_t && it + 1 < ctl->max_it)) ctl->done = 1u;
-        } else {
-            *Dt_next = dtn;
-        }
-    }
+    block_max_to_slot(red, key);
 }
 
 // Ghost fill: X planes (periodic copy or reflective mirror), hydro2d/main.cxx:349-378.
@@ -289,7 +274,7 @@ int g_rows_2d = 64;  // rows per strip (tunable, hydro_tune_set)
 template <typename T>
 int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
                   T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,
-                  hipStream_t s)
+                  hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr)
 {
     if (!rhoE || !uv || !rhoE_next || !uv_next || (!ctl && !Dt_next)) {
         set_error("hydro 2D step: null pointer argument");
@@ -318,6 +303,7 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     }
     const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock);
     const bool wall = bc_y == HYDRO_BC_REFLECTIVE;
+    if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);
     if (ctl) {
         if (wall)
             hipLaunchKernelGGL((fvm2d_kernel<T, true, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
@@ -333,6 +319,14 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
             hipLaunchKernelGGL((fvm2d_kernel<T, false, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
                                DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
     }
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    if (ev_b) HYDRO_HIP_OR(hipEventRecord(ev_b, s), HYDRO_ERR_HIP);
+    if (ctl)
+        hipLaunchKernelGGL((finalize_step_kernel<T, true>), dim3(1), dim3(kWave), 0, s, red, c.C_min, ctl, Dt_next,
+                           traj_t, traj_dt);
+    else
+        hipLaunchKernelGGL((finalize_step_kernel<T, false>), dim3(1), dim3(kWave), 0, s, red, c.C_min, ctl, Dt_next,
+                           traj_t, traj_dt);
     HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
     return 0;
 }
@@ -422,20 +416,21 @@ double timed_launcher2d(const T* d_rhoE, const T* d_uv, T* d_rhoE_next, T* d_uv_
 
 int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
-                    void* traj_dt, hipStream_t s)
+                    void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
     if (precision == 64) {
         const Consts<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step2d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
                                      static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c, nb_x, nb_y,
-                                     0.0, 0.0, bc_y, red, ctl, nullptr, traj_t, static_cast<double*>(traj_dt), s);
+                                     0.0, 0.0, bc_y, red, ctl, nullptr, traj_t, static_cast<double*>(traj_dt), s, ev_a,
+                                     ev_b);
     }
     const Consts<float> c{ static_cast<float>(consts[0]), static_cast<float>(consts[1]),
                            static_cast<float>(consts[2]), static_cast<float>(consts[3]),
                            static_cast<float>(consts[4]) };
     return launch_step2d<float>(static_cast<const float*>(rhoE), static_cast<const float*>(mom),
                                 static_cast<float*>(rhoE_next), static_cast<float*>(mom_next), c, nb_x, nb_y, 0.f, 0.f,
-                                bc_y, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s);
+                                bc_y, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s, ev_a, ev_b);
 }
 
 int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,
@@ -451,6 +446,8 @@ void set_rows_2d(int rows)
 {
     if (rows >= 4 && rows <= 4096) g_rows_2d = rows;
 }
+
+int get_rows_2d() { return g_rows_2d; }
 
 }  // namespace hydro
 

@@ -313,26 +313,7 @@ __global__ void __launch_bounds__(kBlock3) fvm3d_kernel(const T* __restrict__ rh
         mn = mnn;
     }
 
-    const uint64_t wkey = wave_max_u64(key);
-    if (reduce_and_elect(red, wkey, gridDim.x)) {
-        const T dtn = finalize_dt<T>(red, c.C_min);
-        if constexpr (RESIDENT) {
-            const unsigned long long it = ctl->it;
-            if (traj_t) traj_t[it] = ctl->t;
-            if (traj_dt) traj_dt[it] = dtn;
-            const double t_new = ctl->t + static_cast<double>(dtn);  // hydro3d/main.cxx:874-875
-            ctl->dt = static_cast<double>(dtn);
-            ctl->t = t_new;
-            ctl->it = it + 1;
-            if (!isfinite(static_cast<double>(dtn))) {
-                ctl->nonfinite = 1u;
-                ctl->done = 1u;
-            }
-            if (!(t_new < ctl->max_t && it + 1 < ctl->max_it)) ctl->done = 1u;
-        } else {
-            *Dt_next = dtn;
-        }
-    }
+    block_max_to_slot(red, key);
 }
 
 // X ghost planes (hydro3d/main.cxx:374-405), periodic copy or reflective mirror.
@@ -407,7 +388,8 @@ int g_rows_3d = 32;
 template <typename T>
 int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
                   size_t nb_y, size_t nb_z, T DtDx, T DtDy, T DtDz, int bc_y, int bc_z, StepReduce* red,
-                  StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s)
+                  StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
+                  hipEvent_t ev_b = nullptr)
 {
     if (!rhoE || !uvw || !rhoE_next || !uvw_next || (!ctl && !Dt_next)) {
         set_error("hydro 3D step: null pointer argument");
@@ -439,6 +421,7 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     }
     const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock3);
     const bool wy = bc_y == HYDRO_BC_REFLECTIVE, wz = bc_z == HYDRO_BC_REFLECTIVE;
+    if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);
 #define HYDRO_L3(WY, WZ, RES)                                                                                 \
     hipLaunchKernelGGL((fvm3d_kernel<T, WY, WZ, RES>), grid, block, 0, s, rhoE, uvw, rhoE_next, uvw_next, g, c, \
                        DtDx, DtDy, DtDz, red, ctl, Dt_next, traj_t, traj_dt)
@@ -454,6 +437,14 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
         else HYDRO_L3(false, false, false);
     }
 #undef HYDRO_L3
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    if (ev_b) HYDRO_HIP_OR(hipEventRecord(ev_b, s), HYDRO_ERR_HIP);
+    if (ctl)
+        hipLaunchKernelGGL((finalize_step_kernel<T, true>), dim3(1), dim3(kWave), 0, s, red, c.C_min, ctl, Dt_next,
+                           traj_t, traj_dt);
+    else
+        hipLaunchKernelGGL((finalize_step_kernel<T, false>), dim3(1), dim3(kWave), 0, s, red, c.C_min, ctl, Dt_next,
+                           traj_t, traj_dt);
     HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
     return 0;
 }
@@ -549,21 +540,22 @@ double timed_launcher3d(const T* d_rhoE, const T* d_uvw, T* d_rhoE_next, T* d_uv
 
 int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
-                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s)
+                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
     if (precision == 64) {
         const Consts3<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step3d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
                                      static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c, nb_x, nb_y,
                                      nb_z, 0.0, 0.0, 0.0, bc_y, bc_z, red, ctl, nullptr, traj_t,
-                                     static_cast<double*>(traj_dt), s);
+                                     static_cast<double*>(traj_dt), s, ev_a, ev_b);
     }
     const Consts3<float> c{ static_cast<float>(consts[0]), static_cast<float>(consts[1]),
                             static_cast<float>(consts[2]), static_cast<float>(consts[3]),
                             static_cast<float>(consts[4]) };
     return launch_step3d<float>(static_cast<const float*>(rhoE), static_cast<const float*>(mom),
                                 static_cast<float*>(rhoE_next), static_cast<float*>(mom_next), c, nb_x, nb_y, nb_z,
-                                0.f, 0.f, 0.f, bc_y, bc_z, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s);
+                                0.f, 0.f, 0.f, bc_y, bc_z, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s, ev_a,
+                                ev_b);
 }
 
 int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
@@ -580,6 +572,8 @@ void set_tile_3d(int rows)
 {
     if (rows >= 2 && rows <= 4096) g_rows_3d = rows;
 }
+
+int get_tile_3d() { return g_rows_3d; }
 
 }  // namespace hydro
 

@@ -21,6 +21,10 @@ namespace hydro {
 
 constexpr int kWave = 64;
 
+// Sign bit of the (high word of the) floating-point type.
+template <typename T>
+constexpr unsigned int kSignMask = 0x80000000u;
+
 // Primitive state of one cell in the frame of a face: density, normal velocity,
 // tangential velocities, pressure, impedance a = sqrt((gamma*p)*rho).
 template <typename T>
@@ -161,6 +165,17 @@ __device__ __forceinline__ double key_speed(uint64_t k) { return __longlong_as_d
 
 __device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
 
+// x with its sign bit xor-ed with `mask` (0 or the sign bit of the type's high word).
+// Negation is exact, so this yields -x or x bit for bit.
+__device__ __forceinline__ double xor_sign(double x, unsigned int mask)
+{
+    return __hiloint2double(__double2hiint(x) ^ static_cast<int>(mask), __double2loint(x));
+}
+__device__ __forceinline__ float xor_sign(float x, unsigned int mask)
+{
+    return __int_as_float(__float_as_int(x) ^ static_cast<int>(mask));
+}
+
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v)
 {
 #pragma unroll
@@ -185,19 +200,22 @@ __device__ __forceinline__ T from_prev_lane(T x)
     return __shfl_up(x, 1, kWave);
 }
 
-// Per-step reduction state of one solve: running max of the speed keys and a ticket
-// counter that elects the last workgroup to finish, which turns the max into Dt.
+// Per-step reduction state of one solve: running max of the speed keys, spread over
+// kRedSlots words (workgroup b folds into slot b % kRedSlots) so that the atomics of
+// many workgroups do not serialise on one address.  The step kernel only performs
+// atomic max operations; the finalise kernel that follows it in the stream (the kernel
+// boundary orders and publishes the atomics) turns the max into Dt and clears the slots.
+constexpr int kRedSlots = kWave;
+
 struct StepReduce {
-    unsigned long long max_key;
-    unsigned int ticket;
-    unsigned int pad;
+    unsigned long long slot[kRedSlots];
 };
 
 // Device-resident step-loop control of one solve (the host loop of
 // hydro2d/main.cxx:686-815 moved onto the device).  dt is the Dt the next step uses;
-// after each step the elected workgroup stores traj_t[it] = t, traj_dt[it] = Dt_new,
-// then it += 1, t += Dt_new (main.cxx:813-814) and sets done once
-// !(t < max_t && it < max_it).  Steps launched after done are no-ops.
+// after each step the finalise kernel stores traj_t[it] = t, traj_dt[it] = Dt_new, then
+// it += 1, t += Dt_new (main.cxx:813-814) and sets done once !(t < max_t && it < max_it).
+// Steps launched after done are no-ops.
 struct StepControl {
     double dt;
     double t;
@@ -211,36 +229,53 @@ struct StepControl {
     StepReduce red;
 };
 
-// Called by every workgroup once its lanes' keys are folded into `key` (wave-reduced
-// per wave).  The last workgroup to arrive finalises the step.  Returns true in the
-// finalising workgroup's thread 0.
-__device__ __forceinline__ bool reduce_and_elect(StepReduce* red, uint64_t wave_key, unsigned int nblocks)
+// Fold every lane's key into the workgroup maximum and that into the workgroup's slot.
+// Every thread of the workgroup must call it.
+__device__ __forceinline__ void block_max_to_slot(StepReduce* red, uint64_t key)
 {
-    __shared__ unsigned int s_last;
-    if ((threadIdx.x & (kWave - 1)) == 0 && wave_key != 0)
-        atomicMax(&red->max_key, static_cast<unsigned long long>(wave_key));
-    __threadfence();
+    __shared__ unsigned long long s_wave[16];
+    const uint64_t w = wave_max_u64(key);
+    const int nw = static_cast<int>(blockDim.x) / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) s_wave[threadIdx.x / kWave] = w;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned int t = atomicAdd(&red->ticket, 1u);
-        s_last = (t == nblocks - 1) ? 1u : 0u;
+        uint64_t m = 0;
+        for (int q = 0; q < nw; ++q) m = umax64(m, s_wave[q]);
+        if (m != 0) atomicMax(&red->slot[blockIdx.x % kRedSlots], static_cast<unsigned long long>(m));
     }
-    __syncthreads();
-    return s_last && threadIdx.x == 0;
 }
 
-// Turn the reduced max speed into Dt_next = C_min / max_speed (hydro2d/kernel.cxx:318)
-// and clear the reduction state for the next step.
-template <typename T>
-__device__ __forceinline__ T finalize_dt(StepReduce* red, T C_min)
+// Finalise one step (one wave): Dt_next = C_min / max_speed (hydro2d/kernel.cxx:318),
+// clear the slots, and either store Dt_next or advance the resident loop control.
+template <typename T, bool RESIDENT>
+__global__ void __launch_bounds__(kWave) finalize_step_kernel(StepReduce* red, T C_min, StepControl* ctl, T* Dt_next,
+                                                              double* traj_t, T* traj_dt)
 {
-    __threadfence();
-    const unsigned long long k = atomicAdd(&red->max_key, 0ull);
-    const T max_speed = static_cast<T>(key_speed(k));
-    red->max_key = 0ull;
-    red->ticket = 0u;
-    __threadfence();
-    return C_min / max_speed;
+    if constexpr (RESIDENT) {
+        if (ctl->done) return;
+    }
+    const int lane = threadIdx.x;
+    uint64_t k = red->slot[lane];
+    red->slot[lane] = 0ull;
+    k = wave_max_u64(k);
+    if (lane != 0) return;
+    const T dtn = C_min / static_cast<T>(key_speed(k));
+    if constexpr (RESIDENT) {
+        const unsigned long long it = ctl->it;
+        if (traj_t) traj_t[it] = ctl->t;
+        if (traj_dt) traj_dt[it] = dtn;
+        const double t_new = ctl->t + static_cast<double>(dtn);  // hydro2d/main.cxx:813-814
+        ctl->dt = static_cast<double>(dtn);
+        ctl->t = t_new;
+        ctl->it = it + 1;
+        if (!isfinite(static_cast<double>(dtn))) {
+            ctl->nonfinite = 1u;
+            ctl->done = 1u;
+        }
+        if (!(t_new < ctl->max_t && it + 1 < ctl->max_it)) ctl->done = 1u;
+    } else {
+        *Dt_next = dtn;
+    }
 }
 
 }  // namespace hydro

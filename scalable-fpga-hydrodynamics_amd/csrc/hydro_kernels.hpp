@@ -8,13 +8,15 @@
 
 namespace hydro {
 
-// consts = { C_min, gamma, gamma_minus_one, divgamma, K } in the state precision.
+// consts = { C_min, gamma, gamma_minus_one, divgamma, K } in the state precision.  When
+// ev_a / ev_b are given they are recorded around the step kernel (not the finalise).
 int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
-                    void* traj_dt, hipStream_t s);
+                    void* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr);
 int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
-                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s);
+                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
+                    hipEvent_t ev_b = nullptr);
 
 int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,
                 const StepControl* ctl, hipStream_t s);
@@ -23,5 +25,7 @@ int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, in
 
 void set_rows_2d(int rows);
 void set_tile_3d(int rows);
+int get_rows_2d();
+int get_tile_3d();
 
 }  // namespace hydro

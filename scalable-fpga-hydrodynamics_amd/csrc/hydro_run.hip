@@ -12,6 +12,7 @@
 #include <new>
 #include <stdlib.h>
 #include <string.h>
+#include <vector>
 
 #include "../../include/hydro_hip.h"
 #include "hydro_device.hpp"
@@ -31,6 +32,10 @@ struct hydro_run {
     void* d_traj_dt = nullptr;
     size_t max_it = 0;
     size_t enqueued = 0;
+    // Optional per-launch timing of the step kernel (HIP events on the run's stream).
+    bool timing = false;
+    std::vector<hipEvent_t> ev;  // pairs: [2n] before, [2n+1] after launch n
+    size_t ev_used = 0;          // launches recorded since the last query
 };
 
 namespace {
@@ -130,18 +135,30 @@ extern "C" int hydro_run_advance(hydro_run* r, void* rhoE_a, void* mom_a, void* 
         void* nr = in_a ? rhoE_b : rhoE_a;
         void* nm = in_a ? mom_b : mom_a;
         int rc;
+        if (r->timing) {
+            if (r->ev.size() < 2 * (r->ev_used + 1)) {
+                for (int q = 0; q < 2; ++q) {
+                    hipEvent_t e = nullptr;
+                    HYDRO_HIP_OR(hipEventCreate(&e), HYDRO_ERR_HIP);
+                    r->ev.push_back(e);
+                }
+            }
+        }
+        hipEvent_t ev_a = r->timing ? r->ev[2 * r->ev_used] : nullptr;
+        hipEvent_t ev_b = r->timing ? r->ev[2 * r->ev_used + 1] : nullptr;
         if (r->dim == 2) {
             rc = ghost2d_any(cr, cm, r->nb[0], r->nb[1], r->bc[0], r->bc[1], r->precision, r->d_ctl, s);
             if (!rc)
                 rc = step2d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->bc[1], r->precision, r->consts, r->d_ctl,
-                                     &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s);
+                                     &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b);
         } else {
             rc = ghost3d_any(cr, cm, r->nb[0], r->nb[1], r->nb[2], r->bc[0], r->bc[1], r->bc[2], r->precision,
                              r->d_ctl, s);
             if (!rc)
                 rc = step3d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->nb[2], r->bc[1], r->bc[2], r->precision,
-                                     r->consts, r->d_ctl, &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s);
+                                     r->consts, r->d_ctl, &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b);
         }
+        if (!rc && ev_b) ++r->ev_used;
         if (rc) return rc;
         ++r->enqueued;
     }
@@ -214,9 +231,42 @@ extern "C" int hydro_run_trajectory(hydro_run* r, size_t from, size_t count, dou
     return 0;
 }
 
+extern "C" int hydro_run_set_timing(hydro_run* r, int enable)
+{
+    clear_error();
+    if (!r) {
+        set_error("hydro_run_set_timing: null run");
+        return HYDRO_ERR_ARG;
+    }
+    r->timing = enable != 0;
+    r->ev_used = 0;
+    return 0;
+}
+
+extern "C" int hydro_run_kernel_time(hydro_run* r, double* total_ms, size_t* launches)
+{
+    clear_error();
+    if (!r || !total_ms) {
+        set_error("hydro_run_kernel_time: null argument");
+        return HYDRO_ERR_ARG;
+    }
+    double sum = 0.0;
+    for (size_t n = 0; n < r->ev_used; ++n) {
+        HYDRO_HIP_OR(hipEventSynchronize(r->ev[2 * n + 1]), HYDRO_ERR_HIP);
+        float ms = 0.f;
+        HYDRO_HIP_OR(hipEventElapsedTime(&ms, r->ev[2 * n], r->ev[2 * n + 1]), HYDRO_ERR_HIP);
+        sum += ms;
+    }
+    *total_ms = sum;
+    if (launches) *launches = r->ev_used;
+    r->ev_used = 0;
+    return 0;
+}
+
 extern "C" void hydro_run_destroy(hydro_run* r)
 {
     if (!r) return;
+    for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
     if (r->d_ctl) (void)hipFree(r->d_ctl);
     if (r->d_traj_t) (void)hipFree(r->d_traj_t);
     if (r->d_traj_dt) (void)hipFree(r->d_traj_dt);

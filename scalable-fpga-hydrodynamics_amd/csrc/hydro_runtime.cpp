@@ -40,3 +40,29 @@ int current_device()
 extern "C" const char* hydro_last_error(void) { return g_last_error; }
 
 extern "C" int hydro_version(void) { return HYDRO_HIP_ABI_VERSION; }
+
+#include <string.h>
+
+#include "hydro_kernels.hpp"
+
+extern "C" int hydro_tune_set(const char* key, int value)
+{
+    hydro::clear_error();
+    if (key && strcmp(key, "rows2d") == 0 && value >= 4 && value <= 4096) {
+        hydro::set_rows_2d(value);
+        return 0;
+    }
+    if (key && strcmp(key, "rows3d") == 0 && value >= 2 && value <= 4096) {
+        hydro::set_tile_3d(value);
+        return 0;
+    }
+    hydro::set_error("hydro_tune_set: unknown key or value out of range (%s = %d)", key ? key : "(null)", value);
+    return HYDRO_ERR_ARG;
+}
+
+extern "C" int hydro_tune_get(const char* key)
+{
+    if (key && strcmp(key, "rows2d") == 0) return hydro::get_rows_2d();
+    if (key && strcmp(key, "rows3d") == 0) return hydro::get_tile_3d();
+    return HYDRO_ERR_ARG;
+}

@@ -1,0 +1,78 @@
+"""C ABI checks that need no GPU: the library loads, exports every function declared in
+include/hydro_hip.h, and rejects bad arguments with an error code and a message before
+touching the device."""
+import ctypes as C
+import pathlib
+import re
+import subprocess
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "hydro_hip.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*?\**\s*\**([A-Za-z_][A-Za-z0-9_]*)\s*\(", text,
+                       flags=re.M)
+    return sorted({n for n in names if n not in {"if", "while", "for", "return", "sizeof"}})
+
+
+def exported_symbols(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", str(path)], check=True, capture_output=True, text=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_header_declares_the_drop_in_launchers():
+    names = declared_functions()
+    for n in ("launcher", "launcher3d", "hydro_ghost_fill2d", "hydro_ghost_fill3d", "hydro_run_create",
+              "hydro_ic", "hydro_last_error"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol(hydro):
+    lib_path = pathlib.Path(hydro.library_path())
+    assert lib_path.exists(), "libhydro_hip.so is not built"
+    exported = exported_symbols(lib_path)
+    missing = [n for n in declared_functions() if n not in exported]
+    assert not missing, f"declared in hydro_hip.h but not exported: {missing}"
+
+
+def test_header_compiles_as_c(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text('#include "hydro_hip.h"\nint main(void) { return hydro_version() == HYDRO_HIP_ABI_VERSION ? 0 : 1; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", str(ROOT / "include"), "-c", str(src), "-o",
+                    str(tmp_path / "t.o")], check=True)
+
+
+def test_version_and_error_channel(hydro):
+    L = hydro.lib()
+    assert L.hydro_version() == 1
+    assert isinstance(L.hydro_last_error(), bytes)
+
+
+def test_bad_arguments_are_rejected_without_a_device(hydro):
+    L = hydro.lib()
+    sizes = (C.c_size_t * 3)(10, 10, 10)
+    D = (C.c_double * 3)(0.1, 0.1, 0.1)
+    assert L.hydro_run_create(4, sizes, D, 0.05, 1e-3, 1.0, 10, 64, None) is None
+    assert "bad dim" in hydro.last_error()
+    assert L.hydro_run_create(2, sizes, D, 0.05, 1e-3, 1.0, 10, 16, None) is None
+    bad_bc = (C.c_int * 3)(0, 7, 0)
+    assert L.hydro_run_create(2, sizes, D, 0.05, 1e-3, 1.0, 10, 64, bad_bc) is None
+    assert "boundary" in hydro.last_error()
+    with pytest.raises(hydro.HydroError):
+        hydro.tune_set("no_such_knob", 3)
+    with pytest.raises(hydro.HydroError):
+        hydro.tune_set("rows2d", 1)
+    assert L.hydro_run_advance(None, None, None, None, None, 1, None) == -1
+    assert L.hydro_run_trajectory(None, 0, 1, None, None, None) == -1
+
+
+def test_tuning_knobs_round_trip(hydro):
+    old = hydro.tune_get("rows2d")
+    hydro.tune_set("rows2d", 96)
+    assert hydro.tune_get("rows2d") == 96
+    hydro.tune_set("rows2d", old)
+    assert hydro.tune_get("rows3d") >= 2
