@@ -1,0 +1,73 @@
+"""The command-line drivers (`hydro`, `hydro3d`) against the reference's step loop run
+on the CPU oracle: the same stdout lines (initial mass and Dt, per-iteration t and Dt,
+mass lines) and byte-identical CSV files, for the reference's own small configuration
+(hydro2d README example: 60 x 60, 200 iterations, t_max 2, write every 10) and for the
+added boundary-mode and precision options."""
+import pathlib
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import reference_loop
+
+pytestmark = pytest.mark.gpu
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+PKG = ROOT / "scalable-fpga-hydrodynamics_amd"
+
+KEEP = re.compile(r"^(Initial total mass .*|Initial Dt .*|\[it=\s*\d+\] t=.*|\[it=\s*\d+\] hydro_fvm compute time "
+                  r"t=\S+ \(Dt=\S+\))")
+
+
+def _driver_lines(stdout: str):
+    out = []
+    for line in stdout.splitlines():
+        m = KEEP.match(line)
+        if m:
+            out.append(m.group(1))
+    return out
+
+
+def _run(binary, args, cwd):
+    exe = PKG / binary
+    assert exe.exists(), f"{exe} is not built"
+    p = subprocess.run([str(exe), *args], cwd=cwd, capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr[-2000:]
+    return p.stdout
+
+
+CASES = [
+    ("hydro", (60, 60), 2.0, 200, 10, ["-j", "2", "-k", "1"], np.float64, None),
+    ("hydro", (60, 60), 2.0, 200, 10, ["--sync-steps"], np.float64, None),
+    ("hydro", (50, 40), 2.0, 60, 20, ["--precision", "fp32"], np.float32, None),
+    ("hydro", (40, 30), 2.0, 50, 25, ["--bcx", "reflective", "--bcy", "periodic"], np.float64, (1, 0)),
+    ("hydro", (70, 33), 0.01, 1000, 5, [], np.float64, None),
+    ("hydro3d", (24, 20, 16), 2.0, 40, 15, [], np.float64, None),
+    ("hydro3d", (16, 12, 20), 2.0, 30, 10, ["--sync-steps", "--bcz", "periodic"], np.float64, (0, 1, 0)),
+]
+
+
+@pytest.mark.parametrize("binary,nb,max_t,max_it,w,extra,dtype,bc", CASES)
+def test_driver_matches_reference_loop(gpu, oracle, tmp_path, binary, nb, max_t, max_it, w, extra, dtype, bc):
+    args = ["-x", str(nb[0]), "-y", str(nb[1])] + (["-z", str(nb[2])] if len(nb) == 3 else [])
+    args += ["-t", str(max_t), "-i", str(max_it), "-w", str(w), "-o", "gpu"] + extra
+    stdout = _run(binary, args, tmp_path)
+    ref_dir = tmp_path / "ref"
+    ref_dir.mkdir()
+    want_lines, want_files = reference_loop.run(oracle, nb, max_t, max_it, w, ref_dir / "gpu", dtype=dtype, bc=bc)
+    got_lines = _driver_lines(stdout)
+    assert got_lines == want_lines
+    got_files = sorted(p.name for p in tmp_path.glob("gpu_*.csv"))
+    assert got_files == sorted(p.name for p in want_files)
+    for f in want_files:
+        assert (tmp_path / f.name).read_bytes() == f.read_bytes(), f.name
+
+
+def test_driver_help_and_bad_option(gpu, tmp_path):
+    exe = PKG / "hydro"
+    p = subprocess.run([str(exe), "-h"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0 and "Hydrodynamics 2D" in p.stdout and "--bcx" in p.stdout
+    p = subprocess.run([str(exe), "--bcy", "sideways"], capture_output=True, text=True, timeout=60)
+    assert p.returncode != 0 and "Unknown boundary mode" in p.stderr
