@@ -27,11 +27,14 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
 
 WORKLOADS = {
-    # name: (dim, nb, C, ic_mode, radius_frac, p_in, p_out, bytes per cell update)
-    "blast2d_40000x20000_fp64": (2, (40000, 20000), 0.5, 0, 0.0, 10.0, 0.1, 64),
-    "blast3d_512_fp64": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80),
+    # name: (dim, nb, C, ic_mode, radius_frac, p_in, p_out, bytes per cell update, precision)
+    "blast2d_40000x20000_fp64": (2, (40000, 20000), 0.5, 0, 0.0, 10.0, 0.1, 64, 64),
+    "blast3d_512_fp64": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80, 64),
+    "blast2d_40000x20000_fp32flux": (2, (40000, 20000), 0.5, 0, 0.0, 10.0, 0.1, 64, 3264),
+    "blast3d_512_fp32flux": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80, 3264),
 }
-ALIASES = {"2d": "blast2d_40000x20000_fp64", "3d": "blast3d_512_fp64"}
+ALIASES = {"2d": "blast2d_40000x20000_fp64", "3d": "blast3d_512_fp64", "2d-fp32flux": "blast2d_40000x20000_fp32flux",
+           "3d-fp32flux": "blast3d_512_fp32flux"}
 
 
 def log(*a):
@@ -98,6 +101,7 @@ def main() -> int:
     ap.add_argument("--workload", default="2d")
     ap.add_argument("--rows", type=int, default=0, help="override the kernel's X rows per strip")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-copy-reference", action="store_true", help="skip the achievable-bandwidth copy")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -111,7 +115,7 @@ def main() -> int:
     if name not in WORKLOADS:
         log(f"unknown workload {args.workload}; choose from {sorted(WORKLOADS)}")
         return 2
-    dim, nb, C_cfl, ic_mode, rfrac, p_in, p_out, bpc = WORKLOADS[name]
+    dim, nb, C_cfl, ic_mode, rfrac, p_in, p_out, bpc, precision = WORKLOADS[name]
 
     import numpy as np
     import torch
@@ -138,7 +142,7 @@ def main() -> int:
     log(f"[bench] IC done: dt0={dt0:.3e} mass0={mass0:.7e}")
 
     W, K = args.warmup, args.steps
-    run = hydro.Run(nb, k["D"], k["C_min"], dt0, 2.0, W + K)
+    run = hydro.Run(nb, k["D"], k["C_min"], dt0, 2.0, W + K, precision)
     run.advance(a, b, W)
     st = run.status()
     if st["rc"] != 0:
@@ -171,13 +175,31 @@ def main() -> int:
     log(f"[bench] {K} steps: wall {wall * 1e3:.1f} ms, events {gpu_s * 1e3:.1f} ms, kernel avg "
         f"{avg_kernel_s * 1e3:.3f} ms x {launches}, {gcups:.2f} GCUPS, {achieved:.0f} GB/s")
 
+    # Achievable bandwidth: the library's stream copy over the same number of bytes one
+    # step reads (and writes), on buffers of the state's size.  Reuses the state's
+    # second buffer pair as the destination after the run.
+    copy_gbs = None
+    if not args.no_copy_reference:
+        src, dst = cur[0], (b if st["current"] == 0 else a)[0]
+        hydro.stream_copy(dst, src)
+        torch.cuda.synchronize()
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(3):
+            hydro.stream_copy(dst, src)
+        c1.record()
+        torch.cuda.synchronize()
+        nbytes = src.numel() * src.element_size()
+        copy_gbs = 2 * nbytes / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
+        log(f"[bench] stream copy {nbytes / 1e9:.1f} GB: {copy_gbs:.0f} GB/s")
+
     if args.calibrate_copy:
-        # A plain device-to-device copy of exactly 1 GiB (read 1 GiB, write 1 GiB): what
-        # the FETCH_SIZE / WRITE_SIZE counters read for it scales the kernel's counts.
+        # A plain copy of exactly 1 GiB (read 1 GiB, write 1 GiB): what the FETCH_SIZE /
+        # WRITE_SIZE counters read for it scales the step kernel's counts.
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            hydro.stream_copy(dst, src)
         torch.cuda.synchronize()
         del src, dst
 
@@ -198,7 +220,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if precision == 64 else "f64 state, f32 flux",
         "data": "synthetic (built-in blast IC generated on the device)",
         "config": {"workload": name, "grid": list(nb), "cells": cells, "boundaries": "periodic X, reflective Y/Z",
                    "steps_timed": K, "kernel": "fvm2d" if dim == 2 else "fvm3d",
@@ -207,6 +229,8 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(name)},
         "cpu_baseline": cpu,
         "hbm_frac_whole_step": round(bpc * cells * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+        "achievable_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+        "frac_of_achievable": round(achieved / copy_gbs, 4) if copy_gbs else None,
         "kernel_ms": round(avg_kernel_s * 1e3, 4),
         "mass_change_per_cell": abs(mass1 - mass0) / cells,
         "t_final": st["t"],

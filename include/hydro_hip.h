@@ -27,6 +27,13 @@
 #define HYDRO_BC_PERIODIC 0
 #define HYDRO_BC_REFLECTIVE 1
 
+/* State / arithmetic precision of the run and batch APIs. */
+#define HYDRO_FP64 64      /* the reference scheme in FP64 (bit-identical) */
+#define HYDRO_FP32 32      /* the reference's USE_FLOAT build (bit-identical) */
+#define HYDRO_FP32_FLUX 3264 /* FP64 state; FP32 primitives, fluxes and signal speeds with
+                              * hardware reciprocal / square root; increments, update, Dt and
+                              * reductions in FP64 (compared to FP64 with a tolerance) */
+
 #define HYDRO_ERR_ARG (-1)
 #define HYDRO_ERR_HIP (-2)
 #define HYDRO_ERR_NONFINITE (-3)
@@ -111,7 +118,7 @@ typedef struct hydro_run hydro_run;
  * reference computes them (ONE / (NB + 1), hydro2d/main.cxx:482-483); C_min = C *
  * min(D); dt0 the initial Dt; the loop runs while t < max_t and it < max_it.  The
  * per-iteration (t, Dt) pairs of up to max_it steps are recorded on the device.
- * precision: 64 = FP64 (reference), 32 = the USE_FLOAT build.  Returns NULL on error. */
+ * precision: HYDRO_FP64, HYDRO_FP32 or HYDRO_FP32_FLUX.  Returns NULL on error. */
 hydro_run* hydro_run_create(int dim, const size_t* nb, const double* D, double C_min, double dt0, double max_t,
                             size_t max_it, int precision, const int* bc);
 
@@ -173,6 +180,11 @@ int hydro_ic_f32(float* d_rhoE, float* d_mom, int dim, const size_t* nb, const s
  * (hydro2d/main.cxx:237-265).  nb_z = 0 for 2D.  precision 64 or 32. */
 int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision, double* mass,
                      void* stream);
+
+/* Copy `bytes` (a multiple of 16) from src to dst on the device with a plain streaming
+ * kernel: the achievable-bandwidth reference for the step kernels, which read and write
+ * the state once.  No counterpart in the reference (measurement only). */
+int hydro_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 const char* hydro_last_error(void);
 int hydro_version(void);

@@ -18,10 +18,16 @@ import pathlib
 import subprocess
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB_PATH = HERE / "libhydro_hip.so"
+# HYDRO_LIB_PATH selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = pathlib.Path(os.environ.get("HYDRO_LIB_PATH", HERE / "libhydro_hip.so"))
 
 BC_PERIODIC = 0
 BC_REFLECTIVE = 1
+
+# Precisions of the run / batch APIs (include/hydro_hip.h)
+FP64 = 64
+FP32 = 32
+FP32_FLUX = 3264
 
 GAMMA = 1.4
 K = 1.1
@@ -79,6 +85,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_tune_set.argtypes = [C.c_char_p, i32]
     L.hydro_tune_get.argtypes = [C.c_char_p]
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
+    L.hydro_stream_copy.argtypes = [vp, vp, sz, vp]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
 
@@ -276,6 +283,12 @@ class Run:
             self.close()
         except Exception:
             pass
+
+
+def stream_copy(dst, src, stream=None) -> None:
+    """Plain device copy (achievable-bandwidth reference)."""
+    nbytes = src.numel() * src.element_size()
+    _check(lib().hydro_stream_copy(_p(dst), _p(src), nbytes, _stream(stream)), "hydro_stream_copy")
 
 
 def tune_set(key: str, value: int) -> None:

@@ -36,6 +36,18 @@ namespace {
 
 constexpr int kOwned = kWave - 2;  // owned columns per wave window
 constexpr int kWavesPerBlock = 4;
+#ifndef HYDRO_2D_MIN_BLOCKS
+#define HYDRO_2D_MIN_BLOCKS 5
+#endif
+// Rows in flight ahead of the row being converted (register prefetch ring depth).
+#ifndef HYDRO_2D_PREFETCH
+#define HYDRO_2D_PREFETCH 3
+#endif
+constexpr int kPF = HYDRO_2D_PREFETCH;
+// Non-temporal (streaming) stores of the new state.
+#ifndef HYDRO_2D_NT_STORE
+#define HYDRO_2D_NT_STORE 0
+#endif
 constexpr int kBlock = kWave * kWavesPerBlock;
 
 template <typename T>
@@ -48,6 +60,23 @@ template <>
 struct Vec2<float> {
     using type = float2;
 };
+
+template <typename T>
+__device__ __forceinline__ void store_pair(typename Vec2<T>::type* p, T a, T b)
+{
+    if constexpr (HYDRO_2D_NT_STORE) {
+        typedef T v2t __attribute__((ext_vector_type(2)));
+        v2t v;
+        v.x = a;
+        v.y = b;
+        __builtin_nontemporal_store(v, reinterpret_cast<v2t*>(p));
+    } else {
+        typename Vec2<T>::type v;
+        v.x = a;
+        v.y = b;
+        *p = v;
+    }
+}
 
 struct Geom2 {
     int nb_x, nb_y;
@@ -82,20 +111,26 @@ __device__ __forceinline__ void load_cell(const typename Vec2<T>::type* __restri
     b = mo[n];
 }
 
-template <typename T, bool WALL_Y, bool RESIDENT>
-__global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rhoE, const T* __restrict__ uv,
-                                                       T* __restrict__ rhoE_next, T* __restrict__ uv_next, Geom2 g,
-                                                       Consts<T> c, T DtDx_arg, T DtDy_arg, StepReduce* red,
-                                                       StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt)
+// TS: state and update precision; TF, M: precision and math policy of the primitives,
+// fluxes and signal speeds.  <double, double, IeeeMath> is the reference FP64 scheme,
+// <float, float, IeeeMath> its USE_FLOAT build, <double, float, FastMath> the FP32-flux
+// form (FP32 fluxes, increments and update accumulated in FP64).
+template <typename TS, typename TF, typename M, bool WALL_Y, bool RESIDENT>
+__global__ void __launch_bounds__(kBlock, HYDRO_2D_MIN_BLOCKS)
+    fvm2d_kernel(const TS* __restrict__ rhoE, const TS* __restrict__ uv, TS* __restrict__ rhoE_next,
+                 TS* __restrict__ uv_next, Geom2 g, Consts<TS> cs, TS DtDx_arg, TS DtDy_arg, StepReduce* red,
+                 StepControl* ctl, TS* Dt_next, double* traj_t, TS* traj_dt)
 {
-    using V2 = typename Vec2<T>::type;
-    T DtDx = DtDx_arg, DtDy = DtDy_arg;
+    using V2 = typename Vec2<TS>::type;
+    TS DtDx = DtDx_arg, DtDy = DtDy_arg;
     if constexpr (RESIDENT) {
         if (ctl->done) return;  // uniform: the loop has ended, this step is a no-op
-        const T dt = static_cast<T>(ctl->dt);
-        DtDx = dt / static_cast<T>(ctl->D[0]);  // hydro2d/main.cxx:703-704
-        DtDy = dt / static_cast<T>(ctl->D[1]);
+        const TS dt = static_cast<TS>(ctl->dt);
+        DtDx = dt / static_cast<TS>(ctl->D[0]);  // hydro2d/main.cxx:703-704
+        DtDy = dt / static_cast<TS>(ctl->D[1]);
     }
+    const TF gamma = static_cast<TF>(cs.gamma), gmo = static_cast<TF>(cs.gmo);
+    const TF divgamma = static_cast<TF>(cs.divgamma), K = static_cast<TF>(cs.K);
     const V2* __restrict__ R2 = reinterpret_cast<const V2*>(rhoE);
     const V2* __restrict__ M2 = reinterpret_cast<const V2*>(uv);
     V2* __restrict__ RN2 = reinterpret_cast<V2*>(rhoE_next);
@@ -120,7 +155,7 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
     unsigned int ymask = 0u;
     if constexpr (WALL_Y) {
         jl = j < 1 ? 1 : (j > g.nb_y ? g.nb_y : j);
-        ymask = (j == 0 || j == g.nb_y + 1) ? kSignMask<T> : 0u;
+        ymask = (j == 0 || j == g.nb_y + 1) ? kSignMask<TF> : 0u;
     } else {
         jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
     }
@@ -129,31 +164,34 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
 
     uint64_t key = 0;
     if (win < g.nwin) {
-        // Row i0-1 and i0 primitives; X face (i0-1, i0).
-        V2 ra, ma, rc, mc, rn, mn;
-        load_cell<T>(R2, M2, static_cast<long long>(i0 - 1) * g.ys + jl, ra, ma);
-        load_cell<T>(R2, M2, static_cast<long long>(i0) * g.ys + jl, rc, mc);
-        load_cell<T>(R2, M2, static_cast<long long>(i0 + 1) * g.ys + jl, rn, mn);
-        Prim2<T> Pm, Pc;  // X frame: un = u, ut = v
-        Pm.r = ra.x;
-        cons_to_prim2<T>(ra.x, ma.x, ma.y, ra.y, c.gmo, c.gamma, Pm.un, Pm.ut, Pm.p, Pm.a);
-        Pc.r = rc.x;
-        cons_to_prim2<T>(rc.x, mc.x, mc.y, rc.y, c.gmo, c.gamma, Pc.un, Pc.ut, Pc.p, Pc.a);
-        T fx1_0, fx1_1, fx1_2, fx1_3;
-        face_flux2<T>(Pm, Pc, c.K, c.divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
+        // Row i0-1 and i0 primitives; X face (i0-1, i0).  Rows i0+1 .. i0+kPF are
+        // loaded ahead into a register ring (qr/qm), so that kPF rows per wave are in
+        // flight while a row is computed.
+        V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
+        load_cell<TS>(R2, M2, static_cast<long long>(i0 - 1) * g.ys + jl, ra, ma);
+        load_cell<TS>(R2, M2, static_cast<long long>(i0) * g.ys + jl, rc, mc);
+#pragma unroll
+        for (int q = 0; q < kPF; ++q)
+            load_cell<TS>(R2, M2, static_cast<long long>(min(i0 + 1 + q, i1 + 1)) * g.ys + jl, qr[q], qm[q]);
+        Prim2<TF> Pm, Pc;  // X frame: un = u, ut = v
+        Pm.r = static_cast<TF>(ra.x);
+        cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma,
+                             Pm.un, Pm.ut, Pm.p, Pm.a);
+        Pc.r = static_cast<TF>(rc.x);
+        cons_to_prim2<TF, M>(Pc.r, static_cast<TF>(mc.x), static_cast<TF>(mc.y), static_cast<TF>(rc.y), gmo, gamma,
+                             Pc.un, Pc.ut, Pc.p, Pc.a);
+        TF fx1_0, fx1_1, fx1_2, fx1_3;
+        face_flux2<TF, M>(Pm, Pc, K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
 
-        // The loop body has no branch except the final store, so the compiler can
-        // overlap the independent work of consecutive rows.
-#pragma unroll 2
         for (int i = i0; i <= i1; ++i) {
-            // Prefetch row i+2 (clamped to the last row this strip reads).
-            V2 rnn, mnn;
-            const int ipf = min(i + 2, i1 + 1);
-            load_cell<T>(R2, M2, static_cast<long long>(ipf) * g.ys + jl, rnn, mnn);
+            // Prefetch row i+1+kPF (clamped to the last row this strip reads).
+            const int ipf = min(i + 1 + kPF, i1 + 1);
+            load_cell<TS>(R2, M2, static_cast<long long>(ipf) * g.ys + jl, qr[kPF], qm[kPF]);
+            const V2 rn = qr[0], mn = qm[0];  // row i+1
 
             // Y face (j, j+1) of row i in the (rho, v, u, p) frame (kernel.cxx:213-251),
             // evaluated by lane j and handed to lane j+1.
-            Prim2<T> Sy, Ny;
+            Prim2<TF> Sy, Ny;
             Sy.r = Pc.r;
             Sy.un = xor_sign(Pc.ut, ymask);
             Sy.ut = Pc.un;
@@ -164,53 +202,55 @@ __global__ void __launch_bounds__(kBlock) fvm2d_kernel(const T* __restrict__ rho
             Ny.ut = from_next_lane(Sy.ut);
             Ny.p = from_next_lane(Sy.p);
             Ny.a = from_next_lane(Sy.a);
-            T fyR0, fyR1, fyR2, fyR3;
-            face_flux2<T>(Sy, Ny, c.K, c.divgamma, fyR0, fyR1, fyR2, fyR3);
-            const T fyL0 = from_prev_lane(fyR0);
-            const T fyL1 = from_prev_lane(fyR1);
-            const T fyL2 = from_prev_lane(fyR2);
-            const T fyL3 = from_prev_lane(fyR3);
+            TF fyR0, fyR1, fyR2, fyR3;
+            face_flux2<TF, M>(Sy, Ny, K, divgamma, fyR0, fyR1, fyR2, fyR3);
+            const TF fyL0 = from_prev_lane(fyR0);
+            const TF fyL1 = from_prev_lane(fyR1);
+            const TF fyL2 = from_prev_lane(fyR2);
+            const TF fyL3 = from_prev_lane(fyR3);
 
             // Row i+1 primitives and the X face (i, i+1).
-            Prim2<T> Pn;
-            Pn.r = rn.x;
-            cons_to_prim2<T>(rn.x, mn.x, mn.y, rn.y, c.gmo, c.gamma, Pn.un, Pn.ut, Pn.p, Pn.a);
-            T fx2_0, fx2_1, fx2_2, fx2_3;
-            face_flux2<T>(Pc, Pn, c.K, c.divgamma, fx2_0, fx2_1, fx2_2, fx2_3);
+            Prim2<TF> Pn;
+            Pn.r = static_cast<TF>(rn.x);
+            cons_to_prim2<TF, M>(Pn.r, static_cast<TF>(mn.x), static_cast<TF>(mn.y), static_cast<TF>(rn.y), gmo, gamma,
+                                 Pn.un, Pn.ut, Pn.p, Pn.a);
+            TF fx2_0, fx2_1, fx2_2, fx2_3;
+            face_flux2<TF, M>(Pc, Pn, K, divgamma, fx2_0, fx2_1, fx2_2, fx2_3);
 
             // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the
             // (rho, v, u, E) frame, so rho*u takes component 2 and rho*v component 1.
-            const T x_r = DtDx * (fx1_0 - fx2_0);
-            const T x_u = DtDx * (fx1_1 - fx2_1);
-            const T x_v = DtDx * (fx1_2 - fx2_2);
-            const T x_E = DtDx * (fx1_3 - fx2_3);
-            const T y_r = DtDy * (fyL0 - fyR0);
-            const T y_u = DtDy * (fyL2 - fyR2);
-            const T y_v = DtDy * (fyL1 - fyR1);
-            const T y_E = DtDy * (fyL3 - fyR3);
-            const T nr = rc.x + x_r + y_r;
-            const T nu = mc.x + x_u + y_u;
-            const T nv = mc.y + x_v + y_v;
-            const T nE = rc.y + x_E + y_E;
-            const uint64_t k_new = speed_key(static_cast<double>(signal_speed2<T>(nr, nu, nv, nE, c.gmo, c.gamma)));
+            // Increments and update are in the state precision TS.
+            const TS x_r = DtDx * (static_cast<TS>(fx1_0) - static_cast<TS>(fx2_0));
+            const TS x_u = DtDx * (static_cast<TS>(fx1_1) - static_cast<TS>(fx2_1));
+            const TS x_v = DtDx * (static_cast<TS>(fx1_2) - static_cast<TS>(fx2_2));
+            const TS x_E = DtDx * (static_cast<TS>(fx1_3) - static_cast<TS>(fx2_3));
+            const TS y_r = DtDy * (static_cast<TS>(fyL0) - static_cast<TS>(fyR0));
+            const TS y_u = DtDy * (static_cast<TS>(fyL2) - static_cast<TS>(fyR2));
+            const TS y_v = DtDy * (static_cast<TS>(fyL1) - static_cast<TS>(fyR1));
+            const TS y_E = DtDy * (static_cast<TS>(fyL3) - static_cast<TS>(fyR3));
+            const TS nr = rc.x + x_r + y_r;
+            const TS nu = mc.x + x_u + y_u;
+            const TS nv = mc.y + x_v + y_v;
+            const TS nE = rc.y + x_E + y_E;
+            const TF sp = signal_speed2<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
+                                               static_cast<TF>(nE), gmo, gamma);
+            const uint64_t k_new = speed_key(static_cast<double>(sp));
             key = owned ? umax64(key, k_new) : key;
             if (owned) {
                 const long long n = static_cast<long long>(i) * g.ys + j;
-                V2 o1, o2;
-                o1.x = nr;
-                o1.y = nE;
-                o2.x = nu;
-                o2.y = nv;
-                RN2[n] = o1;
-                MN2[n] = o2;
+                store_pair<TS>(RN2 + n, nr, nE);
+                store_pair<TS>(MN2 + n, nu, nv);
             }
 
             // Advance the march.
             Pc = Pn;
             rc = rn;
             mc = mn;
-            rn = rnn;
-            mn = mnn;
+#pragma unroll
+            for (int q = 0; q < kPF; ++q) {
+                qr[q] = qr[q + 1];
+                qm[q] = qm[q + 1];
+            }
             fx1_0 = fx2_0;
             fx1_1 = fx2_1;
             fx1_2 = fx2_2;
@@ -271,7 +311,7 @@ __global__ void ghost_y2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long 
 
 int g_rows_2d = 64;  // rows per strip (tunable, hydro_tune_set)
 
-template <typename T>
+template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
                   T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,
                   hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr)
@@ -306,17 +346,17 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);
     if (ctl) {
         if (wall)
-            hipLaunchKernelGGL((fvm2d_kernel<T, true, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, true, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
                                T(0), T(0), red, ctl, Dt_next, traj_t, traj_dt);
         else
-            hipLaunchKernelGGL((fvm2d_kernel<T, false, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, false, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
                                T(0), T(0), red, ctl, Dt_next, traj_t, traj_dt);
     } else {
         if (wall)
-            hipLaunchKernelGGL((fvm2d_kernel<T, true, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, true, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
                                DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
         else
-            hipLaunchKernelGGL((fvm2d_kernel<T, false, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
+            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, false, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
                                DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
     }
     HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
@@ -418,6 +458,13 @@ int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mo
                     int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
                     void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
+    if (precision == HYDRO_FP32_FLUX) {
+        const Consts<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
+        return launch_step2d<double, float, FastMath>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
+                                                      static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c,
+                                                      nb_x, nb_y, 0.0, 0.0, bc_y, red, ctl, nullptr, traj_t,
+                                                      static_cast<double*>(traj_dt), s, ev_a, ev_b);
+    }
     if (precision == 64) {
         const Consts<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step2d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
@@ -436,7 +483,7 @@ int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mo
 int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,
                 const StepControl* ctl, hipStream_t s)
 {
-    if (precision == 64)
+    if (precision == 64 || precision == HYDRO_FP32_FLUX)
         return launch_ghost2d<double>(static_cast<double*>(rhoE), static_cast<double*>(mom), nb_x, nb_y, bc_x, bc_y,
                                       ctl, s);
     return launch_ghost2d<float>(static_cast<float*>(rhoE), static_cast<float*>(mom), nb_x, nb_y, bc_x, bc_y, ctl, s);

@@ -87,11 +87,13 @@ __device__ __forceinline__ void load3(const typename Vec2_3<T>::type* __restrict
     m = M3[n];
 }
 
-template <typename T>
-__device__ __forceinline__ void to_prim(const typename Vec2_3<T>::type& re, const Trip<T>& m, T gmo, T gamma, P6<T>& q)
+template <typename TS, typename TF, typename M>
+__device__ __forceinline__ void to_prim(const typename Vec2_3<TS>::type& re, const Trip<TS>& m, TF gmo, TF gamma,
+                                        P6<TF>& q)
 {
-    q.r = re.x;
-    cons_to_prim3<T>(re.x, m.x, m.y, m.z, re.y, gmo, gamma, q.u, q.v, q.w, q.p, q.a);
+    q.r = static_cast<TF>(re.x);
+    cons_to_prim3<TF, M>(q.r, static_cast<TF>(m.x), static_cast<TF>(m.y), static_cast<TF>(m.z), static_cast<TF>(re.y),
+                         gmo, gamma, q.u, q.v, q.w, q.p, q.a);
 }
 
 template <typename T>
@@ -110,30 +112,33 @@ __device__ __forceinline__ Prim3<T> frame_z(const P6<T>& q)
     return Prim3<T>{ q.r, q.w, q.v, q.u, q.p, q.a };  // hydro3d/kernel.cxx:296-298
 }
 
-template <typename T, bool WALL_Y, bool WALL_Z, bool RESIDENT>
-__global__ void __launch_bounds__(kBlock3) fvm3d_kernel(const T* __restrict__ rhoE, const T* __restrict__ uvw,
-                                                        T* __restrict__ rhoE_next, T* __restrict__ uvw_next, Geom3 g,
-                                                        Consts3<T> c, T DtDx_arg, T DtDy_arg, T DtDz_arg,
-                                                        StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t,
-                                                        T* traj_dt)
+// TS: state and update precision; TF, M: precision and math policy of the primitives,
+// fluxes and signal speeds (see the 2D kernel).
+template <typename TS, typename TF, typename M, bool WALL_Y, bool WALL_Z, bool RESIDENT>
+__global__ void __launch_bounds__(kBlock3)
+    fvm3d_kernel(const TS* __restrict__ rhoE, const TS* __restrict__ uvw, TS* __restrict__ rhoE_next,
+                 TS* __restrict__ uvw_next, Geom3 g, Consts3<TS> cs, TS DtDx_arg, TS DtDy_arg, TS DtDz_arg,
+                 StepReduce* red, StepControl* ctl, TS* Dt_next, double* traj_t, TS* traj_dt)
 {
-    using V2 = typename Vec2_3<T>::type;
+    using V2 = typename Vec2_3<TS>::type;
     // Y-frame primitives (rho, v, u, w, p, a) and Y-face fluxes of every wave, per lane.
-    __shared__ T s_prim[kNW][6][kWave];
-    __shared__ T s_flux[kNW][5][kWave];
+    __shared__ TF s_prim[kNW][6][kWave];
+    __shared__ TF s_flux[kNW][5][kWave];
 
-    T DtDx = DtDx_arg, DtDy = DtDy_arg, DtDz = DtDz_arg;
+    TS DtDx = DtDx_arg, DtDy = DtDy_arg, DtDz = DtDz_arg;
     if constexpr (RESIDENT) {
         if (ctl->done) return;
-        const T dt = static_cast<T>(ctl->dt);
-        DtDx = dt / static_cast<T>(ctl->D[0]);  // hydro3d/main.cxx:759-761
-        DtDy = dt / static_cast<T>(ctl->D[1]);
-        DtDz = dt / static_cast<T>(ctl->D[2]);
+        const TS dt = static_cast<TS>(ctl->dt);
+        DtDx = dt / static_cast<TS>(ctl->D[0]);  // hydro3d/main.cxx:759-761
+        DtDy = dt / static_cast<TS>(ctl->D[1]);
+        DtDz = dt / static_cast<TS>(ctl->D[2]);
     }
+    const TF gamma = static_cast<TF>(cs.gamma), gmo = static_cast<TF>(cs.gmo);
+    const TF divgamma = static_cast<TF>(cs.divgamma), K = static_cast<TF>(cs.K);
     const V2* __restrict__ R2 = reinterpret_cast<const V2*>(rhoE);
-    const Trip<T>* __restrict__ M3 = reinterpret_cast<const Trip<T>*>(uvw);
+    const Trip<TS>* __restrict__ M3 = reinterpret_cast<const Trip<TS>*>(uvw);
     V2* __restrict__ RN2 = reinterpret_cast<V2*>(rhoE_next);
-    Trip<T>* __restrict__ MN3 = reinterpret_cast<Trip<T>*>(uvw_next);
+    Trip<TS>* __restrict__ MN3 = reinterpret_cast<Trip<TS>*>(uvw_next);
 
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
@@ -147,68 +152,79 @@ __global__ void __launch_bounds__(kBlock3) fvm3d_kernel(const T* __restrict__ rh
     const int j = yt * kTY + w;  // row of this wave (j0 - 1 + w with j0 = 1 + yt*TY)
     const int k = zw * kOwnedZ + lane;
     const bool halo_wave = (w == 0) || (w == kNW - 1);
-    const bool row_in = j <= g.nb_y;
-    const bool owned = !halo_wave && row_in && lane >= 1 && lane <= kOwnedZ && k <= g.nb_z;
-    const int jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
-    const int kl = k <= g.nb_z + 1 ? k : g.nb_z + 1;
+    const bool owned = !halo_wave && j <= g.nb_y && lane >= 1 && lane <= kOwnedZ && k <= g.nb_z;
+    // Reflective walls as in 2D: a ghost row (wave) / ghost column (lane) loads its
+    // interior neighbour and presents its mirror, normal velocity negated, to the face
+    // (hydro3d/kernel.cxx:248-252, 269-273, 290-294, 311-315).
+    int jl, kl;
+    unsigned int ymask = 0u, zmask = 0u;
+    if constexpr (WALL_Y) {
+        jl = j < 1 ? 1 : (j > g.nb_y ? g.nb_y : j);
+        ymask = (j == 0 || j == g.nb_y + 1) ? kSignMask<TF> : 0u;
+    } else {
+        jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
+    }
+    if constexpr (WALL_Z) {
+        kl = k < 1 ? 1 : (k > g.nb_z ? g.nb_z : k);
+        zmask = (k == 0 || k == g.nb_z + 1) ? kSignMask<TF> : 0u;
+    } else {
+        kl = k <= g.nb_z + 1 ? k : g.nb_z + 1;
+    }
     const long long col = static_cast<long long>(jl) * g.zs + kl;
     const int i0 = 1 + strip * g.rows;
     const int i1 = min(i0 + g.rows - 1, g.nb_x);
 
     uint64_t key = 0;
     V2 rc, rn;
-    Trip<T> mc, mn;
-    P6<T> Pc;
-    T fx1_0 = T(0), fx1_1 = T(0), fx1_2 = T(0), fx1_3 = T(0), fx1_4 = T(0);
-    load3<T>(R2, M3, static_cast<long long>(i0) * g.zy + col, rc, mc);
-    load3<T>(R2, M3, static_cast<long long>(i0 + 1) * g.zy + col, rn, mn);
-    to_prim<T>(rc, mc, c.gmo, c.gamma, Pc);
+    Trip<TS> mc, mn;
+    P6<TF> Pc;
+    TF fx1_0 = TF(0), fx1_1 = TF(0), fx1_2 = TF(0), fx1_3 = TF(0), fx1_4 = TF(0);
+    load3<TS>(R2, M3, static_cast<long long>(i0) * g.zy + col, rc, mc);
+    load3<TS>(R2, M3, static_cast<long long>(i0 + 1) * g.zy + col, rn, mn);
+    to_prim<TS, TF, M>(rc, mc, gmo, gamma, Pc);
     if (!halo_wave) {
         V2 ra;
-        Trip<T> ma;
-        P6<T> Pm;
-        load3<T>(R2, M3, static_cast<long long>(i0 - 1) * g.zy + col, ra, ma);
-        to_prim<T>(ra, ma, c.gmo, c.gamma, Pm);
-        face_flux3<T>(frame_x(Pm), frame_x(Pc), c.K, c.divgamma, fx1_0, fx1_1, fx1_2, fx1_3, fx1_4);
+        Trip<TS> ma;
+        P6<TF> Pm;
+        load3<TS>(R2, M3, static_cast<long long>(i0 - 1) * g.zy + col, ra, ma);
+        to_prim<TS, TF, M>(ra, ma, gmo, gamma, Pm);
+        face_flux3<TF, M>(frame_x(Pm), frame_x(Pc), K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3, fx1_4);
     }
 
     for (int i = i0; i <= i1; ++i) {
         V2 rnn;
-        Trip<T> mnn;
+        Trip<TS> mnn;
         const int ipf = min(i + 2, i1 + 1);
-        load3<T>(R2, M3, static_cast<long long>(ipf) * g.zy + col, rnn, mnn);
+        load3<TS>(R2, M3, static_cast<long long>(ipf) * g.zy + col, rnn, mnn);
 
         // (a) publish this row's Y-frame primitives.
-        const Prim3<T> Sy = frame_y(Pc);
+        const Prim3<TF> Sy = frame_y(Pc);
         s_prim[w][0][lane] = Sy.r;
-        s_prim[w][1][lane] = Sy.un;
+        s_prim[w][1][lane] = xor_sign(Sy.un, ymask);
         s_prim[w][2][lane] = Sy.ut1;
         s_prim[w][3][lane] = Sy.ut2;
         s_prim[w][4][lane] = Sy.p;
         s_prim[w][5][lane] = Sy.a;
         __syncthreads();
 
-        // (b) Y face (j, j+1), hydro3d/kernel.cxx:239-278, evaluated by the wave of row j.
+        // (b) Y face (j, j+1), hydro3d/kernel.cxx:239-278, evaluated by the wave of row j
+        // from the primitives the wave of row j+1 published.
         if (w < kNW - 1) {
-            Prim3<T> Ny;
-            Ny.r = s_prim[w + 1][0][lane];
-            Ny.un = s_prim[w + 1][1][lane];
-            Ny.ut1 = s_prim[w + 1][2][lane];
-            Ny.ut2 = s_prim[w + 1][3][lane];
-            Ny.p = s_prim[w + 1][4][lane];
-            Ny.a = s_prim[w + 1][5][lane];
-            Prim3<T> L = Sy, R = Ny;
-            if constexpr (WALL_Y) {
-                if (j == g.nb_y) {  // wall below row nb_y: flux(c, mirror(c))
-                    R = Sy;
-                    R.un = -Sy.un;
-                } else if (j == 0) {  // wall above row 1: flux(mirror(c1), c1)
-                    L = Ny;
-                    L.un = -Ny.un;
-                }
-            }
-            T f0, f1, f2, f3, f4;
-            face_flux3<T>(L, R, c.K, c.divgamma, f0, f1, f2, f3, f4);
+            Prim3<TF> L, R;
+            L.r = s_prim[w][0][lane];
+            L.un = s_prim[w][1][lane];
+            L.ut1 = s_prim[w][2][lane];
+            L.ut2 = s_prim[w][3][lane];
+            L.p = s_prim[w][4][lane];
+            L.a = s_prim[w][5][lane];
+            R.r = s_prim[w + 1][0][lane];
+            R.un = s_prim[w + 1][1][lane];
+            R.ut1 = s_prim[w + 1][2][lane];
+            R.ut2 = s_prim[w + 1][3][lane];
+            R.p = s_prim[w + 1][4][lane];
+            R.a = s_prim[w + 1][5][lane];
+            TF f0, f1, f2, f3, f4;
+            face_flux3<TF, M>(L, R, K, divgamma, f0, f1, f2, f3, f4);
             s_flux[w][0][lane] = f0;
             s_flux[w][1][lane] = f1;
             s_flux[w][2][lane] = f2;
@@ -218,87 +234,80 @@ __global__ void __launch_bounds__(kBlock3) fvm3d_kernel(const T* __restrict__ rh
         __syncthreads();
 
         V2 rcur = rc;
-        Trip<T> mcur = mc;
-        P6<T> Pn;
-        to_prim<T>(rn, mn, c.gmo, c.gamma, Pn);
+        Trip<TS> mcur = mc;
+        P6<TF> Pn;
+        to_prim<TS, TF, M>(rn, mn, gmo, gamma, Pn);
         if (!halo_wave) {
             // (c) Y face above this row, from the wave of row j-1.
-            const T fy1_0 = s_flux[w - 1][0][lane];
-            const T fy1_1 = s_flux[w - 1][1][lane];
-            const T fy1_2 = s_flux[w - 1][2][lane];
-            const T fy1_3 = s_flux[w - 1][3][lane];
-            const T fy1_4 = s_flux[w - 1][4][lane];
-            const T fy2_0 = s_flux[w][0][lane];
-            const T fy2_1 = s_flux[w][1][lane];
-            const T fy2_2 = s_flux[w][2][lane];
-            const T fy2_3 = s_flux[w][3][lane];
-            const T fy2_4 = s_flux[w][4][lane];
+            const TF fy1_0 = s_flux[w - 1][0][lane];
+            const TF fy1_1 = s_flux[w - 1][1][lane];
+            const TF fy1_2 = s_flux[w - 1][2][lane];
+            const TF fy1_3 = s_flux[w - 1][3][lane];
+            const TF fy1_4 = s_flux[w - 1][4][lane];
+            const TF fy2_0 = s_flux[w][0][lane];
+            const TF fy2_1 = s_flux[w][1][lane];
+            const TF fy2_2 = s_flux[w][2][lane];
+            const TF fy2_3 = s_flux[w][3][lane];
+            const TF fy2_4 = s_flux[w][4][lane];
 
             // Z face (k, k+1) by lane k, shuffled to lane k+1, hydro3d/kernel.cxx:281-320.
-            const Prim3<T> Sz = frame_z(Pc);
-            Prim3<T> Nz;
+            Prim3<TF> Sz = frame_z(Pc);
+            Sz.un = xor_sign(Sz.un, zmask);
+            Prim3<TF> Nz;
             Nz.r = from_next_lane(Sz.r);
             Nz.un = from_next_lane(Sz.un);
             Nz.ut1 = from_next_lane(Sz.ut1);
             Nz.ut2 = from_next_lane(Sz.ut2);
             Nz.p = from_next_lane(Sz.p);
             Nz.a = from_next_lane(Sz.a);
-            T fz2_0, fz2_1, fz2_2, fz2_3, fz2_4;
-            face_flux3<T>(Sz, Nz, c.K, c.divgamma, fz2_0, fz2_1, fz2_2, fz2_3, fz2_4);
-            if constexpr (WALL_Z) {
-                const bool back = (k == g.nb_z);
-                const bool front = (k == 0);
-                if (back || front) {
-                    Prim3<T> L = back ? Sz : Nz;
-                    Prim3<T> R = L;
-                    if (back) R.un = -Sz.un;
-                    else L.un = -Nz.un;
-                    face_flux3<T>(L, R, c.K, c.divgamma, fz2_0, fz2_1, fz2_2, fz2_3, fz2_4);
-                }
-            }
-            const T fz1_0 = from_prev_lane(fz2_0);
-            const T fz1_1 = from_prev_lane(fz2_1);
-            const T fz1_2 = from_prev_lane(fz2_2);
-            const T fz1_3 = from_prev_lane(fz2_3);
-            const T fz1_4 = from_prev_lane(fz2_4);
+            TF fz2_0, fz2_1, fz2_2, fz2_3, fz2_4;
+            face_flux3<TF, M>(Sz, Nz, K, divgamma, fz2_0, fz2_1, fz2_2, fz2_3, fz2_4);
+            const TF fz1_0 = from_prev_lane(fz2_0);
+            const TF fz1_1 = from_prev_lane(fz2_1);
+            const TF fz1_2 = from_prev_lane(fz2_2);
+            const TF fz1_3 = from_prev_lane(fz2_3);
+            const TF fz1_4 = from_prev_lane(fz2_4);
 
             // X face (i, i+1).
-            T fx2_0, fx2_1, fx2_2, fx2_3, fx2_4;
-            face_flux3<T>(frame_x(Pc), frame_x(Pn), c.K, c.divgamma, fx2_0, fx2_1, fx2_2, fx2_3, fx2_4);
+            TF fx2_0, fx2_1, fx2_2, fx2_3, fx2_4;
+            face_flux3<TF, M>(frame_x(Pc), frame_x(Pn), K, divgamma, fx2_0, fx2_1, fx2_2, fx2_3, fx2_4);
 
-            // Update, hydro3d/kernel.cxx:330-358: U' = ((U + x) + y) + z.
-            const T x_r = DtDx * (fx1_0 - fx2_0);
-            const T x_u = DtDx * (fx1_1 - fx2_1);
-            const T x_v = DtDx * (fx1_2 - fx2_2);
-            const T x_w = DtDx * (fx1_3 - fx2_3);
-            const T x_E = DtDx * (fx1_4 - fx2_4);
-            const T y_r = DtDy * (fy1_0 - fy2_0);
-            const T y_u = DtDy * (fy1_2 - fy2_2);
-            const T y_v = DtDy * (fy1_1 - fy2_1);
-            const T y_w = DtDy * (fy1_3 - fy2_3);
-            const T y_E = DtDy * (fy1_4 - fy2_4);
-            const T z_r = DtDz * (fz1_0 - fz2_0);
-            const T z_u = DtDz * (fz1_3 - fz2_3);
-            const T z_v = DtDz * (fz1_2 - fz2_2);
-            const T z_w = DtDz * (fz1_1 - fz2_1);
-            const T z_E = DtDz * (fz1_4 - fz2_4);
-            const T nr = rcur.x + x_r + y_r + z_r;
-            const T nu = mcur.x + x_u + y_u + z_u;
-            const T nv = mcur.y + x_v + y_v + z_v;
-            const T nw = mcur.z + x_w + y_w + z_w;
-            const T nE = rcur.y + x_E + y_E + z_E;
+            // Update, hydro3d/kernel.cxx:330-358: U' = ((U + x) + y) + z, in TS.
+            const TS x_r = DtDx * (static_cast<TS>(fx1_0) - static_cast<TS>(fx2_0));
+            const TS x_u = DtDx * (static_cast<TS>(fx1_1) - static_cast<TS>(fx2_1));
+            const TS x_v = DtDx * (static_cast<TS>(fx1_2) - static_cast<TS>(fx2_2));
+            const TS x_w = DtDx * (static_cast<TS>(fx1_3) - static_cast<TS>(fx2_3));
+            const TS x_E = DtDx * (static_cast<TS>(fx1_4) - static_cast<TS>(fx2_4));
+            const TS y_r = DtDy * (static_cast<TS>(fy1_0) - static_cast<TS>(fy2_0));
+            const TS y_u = DtDy * (static_cast<TS>(fy1_2) - static_cast<TS>(fy2_2));
+            const TS y_v = DtDy * (static_cast<TS>(fy1_1) - static_cast<TS>(fy2_1));
+            const TS y_w = DtDy * (static_cast<TS>(fy1_3) - static_cast<TS>(fy2_3));
+            const TS y_E = DtDy * (static_cast<TS>(fy1_4) - static_cast<TS>(fy2_4));
+            const TS z_r = DtDz * (static_cast<TS>(fz1_0) - static_cast<TS>(fz2_0));
+            const TS z_u = DtDz * (static_cast<TS>(fz1_3) - static_cast<TS>(fz2_3));
+            const TS z_v = DtDz * (static_cast<TS>(fz1_2) - static_cast<TS>(fz2_2));
+            const TS z_w = DtDz * (static_cast<TS>(fz1_1) - static_cast<TS>(fz2_1));
+            const TS z_E = DtDz * (static_cast<TS>(fz1_4) - static_cast<TS>(fz2_4));
+            const TS nr = rcur.x + x_r + y_r + z_r;
+            const TS nu = mcur.x + x_u + y_u + z_u;
+            const TS nv = mcur.y + x_v + y_v + z_v;
+            const TS nw = mcur.z + x_w + y_w + z_w;
+            const TS nE = rcur.y + x_E + y_E + z_E;
+            const TF sp = signal_speed3<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
+                                               static_cast<TF>(nw), static_cast<TF>(nE), gmo, gamma);
+            const uint64_t k_new = speed_key(static_cast<double>(sp));
+            key = owned ? umax64(key, k_new) : key;
             if (owned) {
                 const long long n = static_cast<long long>(i) * g.zy + col;
                 V2 o;
                 o.x = nr;
                 o.y = nE;
                 RN2[n] = o;
-                Trip<T> m;
+                Trip<TS> m;
                 m.x = nu;
                 m.y = nv;
                 m.z = nw;
                 MN3[n] = m;
-                key = umax64(key, speed_key(static_cast<double>(signal_speed3<T>(nr, nu, nv, nw, nE, c.gmo, c.gamma))));
             }
             fx1_0 = fx2_0;
             fx1_1 = fx2_1;
@@ -385,7 +394,7 @@ __global__ void ghost_z3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long
 
 int g_rows_3d = 32;
 
-template <typename T>
+template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
                   size_t nb_y, size_t nb_z, T DtDx, T DtDy, T DtDz, int bc_y, int bc_z, StepReduce* red,
                   StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
@@ -423,7 +432,7 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     const bool wy = bc_y == HYDRO_BC_REFLECTIVE, wz = bc_z == HYDRO_BC_REFLECTIVE;
     if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);
 #define HYDRO_L3(WY, WZ, RES)                                                                                 \
-    hipLaunchKernelGGL((fvm3d_kernel<T, WY, WZ, RES>), grid, block, 0, s, rhoE, uvw, rhoE_next, uvw_next, g, c, \
+    hipLaunchKernelGGL((fvm3d_kernel<T, TF, M, WY, WZ, RES>), grid, block, 0, s, rhoE, uvw, rhoE_next, uvw_next, g, c, \
                        DtDx, DtDy, DtDz, red, ctl, Dt_next, traj_t, traj_dt)
     if (ctl) {
         if (wy && wz) HYDRO_L3(true, true, true);
@@ -542,6 +551,13 @@ int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mo
                     size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
                     StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
 {
+    if (precision == HYDRO_FP32_FLUX) {
+        const Consts3<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
+        return launch_step3d<double, float, FastMath>(
+            static_cast<const double*>(rhoE), static_cast<const double*>(mom), static_cast<double*>(rhoE_next),
+            static_cast<double*>(mom_next), c, nb_x, nb_y, nb_z, 0.0, 0.0, 0.0, bc_y, bc_z, red, ctl, nullptr, traj_t,
+            static_cast<double*>(traj_dt), s, ev_a, ev_b);
+    }
     if (precision == 64) {
         const Consts3<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step3d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
@@ -561,7 +577,7 @@ int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mo
 int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
                 int precision, const StepControl* ctl, hipStream_t s)
 {
-    if (precision == 64)
+    if (precision == 64 || precision == HYDRO_FP32_FLUX)
         return launch_ghost3d<double>(static_cast<double*>(rhoE), static_cast<double*>(mom), nb_x, nb_y, nb_z, bc_x,
                                       bc_y, bc_z, ctl, s);
     return launch_ghost3d<float>(static_cast<float*>(rhoE), static_cast<float*>(mom), nb_x, nb_y, nb_z, bc_x, bc_y,

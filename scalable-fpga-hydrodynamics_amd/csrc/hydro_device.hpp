@@ -43,42 +43,65 @@ __device__ __forceinline__ T rmax(T a, T b)
     return (a < b) ? b : a;  // std::max(a, b), hydro2d/kernel.hpp:36
 }
 
+// Math policies for the reciprocals and square roots of the scheme.
+//   IeeeMath: correctly rounded ONE / x and sqrt(x), exactly as the reference computes
+//             them (FP64, and FP32 for the USE_FLOAT build).
+//   FastMath: the hardware approximations (v_rcp_f32, v_sqrt_f32, about 1 ulp), for
+//             the FP32-flux form whose results are compared to FP64 with a tolerance.
+struct IeeeMath {
+    template <typename T>
+    static __device__ __forceinline__ T recip(T x)
+    {
+        return T(1) / x;
+    }
+    template <typename T>
+    static __device__ __forceinline__ T root(T x)
+    {
+        return sqrt(x);
+    }
+};
+
+struct FastMath {
+    static __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+
 // Conserved -> primitive, hydro2d/kernel.cxx:5-29; plus the impedance of compute_fluxes
 // (kernel.cxx:92-93), evaluated as sqrt((gamma*p)*rho).
-template <typename T>
+template <typename T, typename M = IeeeMath>
 __device__ __forceinline__ void cons_to_prim2(T rho, T mx, T my, T E, T gmo, T gamma, T& u, T& v, T& p, T& a)
 {
-    const T inv = T(1) / rho;
+    const T inv = M::recip(rho);
     u = mx * inv;
     v = my * inv;
     const T tmp = u * u + v * v;
     const T e = E * inv - T(0.5) * tmp;
     p = gmo * rho * e;
-    a = sqrt(gamma * p * rho);
+    a = M::root(gamma * p * rho);
 }
 
-template <typename T>
+template <typename T, typename M = IeeeMath>
 __device__ __forceinline__ void cons_to_prim3(T rho, T mx, T my, T mz, T E, T gmo, T gamma, T& u, T& v, T& w, T& p,
                                               T& a)
 {
-    const T inv = T(1) / rho;
+    const T inv = M::recip(rho);
     u = mx * inv;
     v = my * inv;
     w = mz * inv;
     const T tmp = u * u + v * v + w * w;
     const T e = E * inv - T(0.5) * tmp;
     p = gmo * rho * e;
-    a = sqrt(gamma * p * rho);
+    a = M::root(gamma * p * rho);
 }
 
 // Face flux from precomputed impedances, hydro2d/kernel.cxx:95-117.
 // F = (mass, normal momentum, tangential momentum, energy) in the face frame.
-template <typename T>
+template <typename T, typename M = IeeeMath>
 __device__ __forceinline__ void face_flux2(const Prim2<T>& L, const Prim2<T>& R, T K, T divgamma, T& F0, T& F1,
                                            T& F2, T& F3)
 {
     const T a = K * rmax(L.a, R.a);
-    const T diva = T(1) / a;
+    const T diva = M::recip(a);
     const T us = T(0.5) * ((L.un + R.un) - diva * (R.p - L.p));
     const T ps = T(0.5) * ((L.p + R.p) - a * (R.un - L.un));
     const bool up = us >= T(0);
@@ -96,12 +119,12 @@ __device__ __forceinline__ void face_flux2(const Prim2<T>& L, const Prim2<T>& R,
 }
 
 // hydro3d/kernel.cxx:101-129.
-template <typename T>
+template <typename T, typename M = IeeeMath>
 __device__ __forceinline__ void face_flux3(const Prim3<T>& L, const Prim3<T>& R, T K, T divgamma, T& F0, T& F1,
                                            T& F2, T& F3, T& F4)
 {
     const T a = K * rmax(L.a, R.a);
-    const T diva = T(1) / a;
+    const T diva = M::recip(a);
     const T us = T(0.5) * ((L.un + R.un) - diva * (R.p - L.p));
     const T ps = T(0.5) * ((L.p + R.p) - a * (R.un - L.un));
     const bool up = us >= T(0);
@@ -124,32 +147,32 @@ __device__ __forceinline__ void face_flux3(const Prim3<T>& L, const Prim3<T>& R,
 }
 
 // Signal speed c_s + |u| of a conserved state, hydro2d/kernel.cxx:287-309.
-template <typename T>
+template <typename T, typename M = IeeeMath>
 __device__ __forceinline__ T signal_speed2(T rho, T mx, T my, T E, T gmo, T gamma)
 {
-    const T inv = T(1) / rho;
+    const T inv = M::recip(rho);
     const T u = mx * inv;
     const T v = my * inv;
     const T tmp = u * u + v * v;
     const T e = E * inv - T(0.5) * tmp;
     const T p = gmo * rho * e;
-    const T cs = sqrt(gamma * p * inv);
-    const T un = sqrt(tmp);
+    const T cs = M::root(gamma * p * inv);
+    const T un = M::root(tmp);
     return cs + un;
 }
 
-template <typename T>
+template <typename T, typename M = IeeeMath>
 __device__ __forceinline__ T signal_speed3(T rho, T mx, T my, T mz, T E, T gmo, T gamma)
 {
-    const T inv = T(1) / rho;
+    const T inv = M::recip(rho);
     const T u = mx * inv;
     const T v = my * inv;
     const T w = mz * inv;
     const T tmp = u * u + v * v + w * w;
     const T e = E * inv - T(0.5) * tmp;
     const T p = gmo * rho * e;
-    const T cs = sqrt(gamma * p * inv);
-    const T un = sqrt(tmp);
+    const T cs = M::root(gamma * p * inv);
+    const T un = M::root(tmp);
     return cs + un;
 }
 

@@ -43,7 +43,7 @@ namespace {
 // gamma, K and the derived constants of hydro2d/main.cxx:601-606 in the state precision.
 void physics_constants(int precision, double C_min, double out[5])
 {
-    if (precision == 64) {
+    if (precision != HYDRO_FP32) {
         const double gamma = 1.4, K = 1.1;
         const double gmo = gamma - 1.0;
         out[0] = C_min;
@@ -68,7 +68,8 @@ extern "C" hydro_run* hydro_run_create(int dim, const size_t* nb, const double* 
                                        double max_t, size_t max_it, int precision, const int* bc)
 {
     clear_error();
-    if ((dim != 2 && dim != 3) || !nb || !D || (precision != 64 && precision != 32)) {
+    if ((dim != 2 && dim != 3) || !nb || !D ||
+        (precision != HYDRO_FP64 && precision != HYDRO_FP32 && precision != HYDRO_FP32_FLUX)) {
         set_error("hydro_run_create: bad dim/precision/pointer arguments");
         return nullptr;
     }
@@ -98,15 +99,16 @@ extern "C" hydro_run* hydro_run_create(int dim, const size_t* nb, const double* 
 
     StepControl h;
     memset(&h, 0, sizeof(h));
-    h.dt = precision == 64 ? dt0 : static_cast<double>(static_cast<float>(dt0));
+    const bool f32_state = precision == HYDRO_FP32;
+    h.dt = f32_state ? static_cast<double>(static_cast<float>(dt0)) : dt0;
     h.t = 0.0;
     h.max_t = max_t;
     h.it = 0;
     h.max_it = max_it;
     h.C_min = r->consts[0];
-    for (int d = 0; d < dim; ++d) h.D[d] = precision == 64 ? D[d] : static_cast<double>(static_cast<float>(D[d]));
+    for (int d = 0; d < dim; ++d) h.D[d] = f32_state ? static_cast<double>(static_cast<float>(D[d])) : D[d];
     h.done = (0.0 < max_t && 0 < max_it) ? 0u : 1u;  // hydro2d/main.cxx:689
-    const size_t traj_bytes_dt = (max_it ? max_it : 1) * (precision == 64 ? 8 : 4);
+    const size_t traj_bytes_dt = (max_it ? max_it : 1) * (f32_state ? 4 : 8);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->d_ctl), sizeof(StepControl));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&r->d_traj_t), (max_it ? max_it : 1) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&r->d_traj_dt, traj_bytes_dt);
@@ -205,7 +207,7 @@ extern "C" int hydro_run_trajectory(hydro_run* r, size_t from, size_t count, dou
         HYDRO_HIP_OR(hipMemcpyAsync(t_out, r->d_traj_t + from, count * sizeof(double), hipMemcpyDeviceToHost, s),
                      HYDRO_ERR_HIP);
     if (dt_out) {
-        if (r->precision == 64) {
+        if (r->precision != HYDRO_FP32) {
             HYDRO_HIP_OR(hipMemcpyAsync(dt_out, static_cast<double*>(r->d_traj_dt) + from, count * sizeof(double),
                                         hipMemcpyDeviceToHost, s),
                          HYDRO_ERR_HIP);
@@ -348,6 +350,7 @@ extern "C" int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, si
                                 double* mass, void* stream)
 {
     clear_error();
+    if (precision == HYDRO_FP32_FLUX) precision = HYDRO_FP64;  // FP64 state
     if (!d_rhoE || !mass || NB_X < 1 || NB_Y < 1 || (precision != 64 && precision != 32)) {
         set_error("hydro_total_mass: bad arguments");
         return HYDRO_ERR_ARG;

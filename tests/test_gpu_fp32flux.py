@@ -1,0 +1,58 @@
+"""The FP32-flux form (FP64 state; FP32 primitives, fluxes and signal speeds with
+hardware reciprocal and square root; FP64 increments, update, Dt and reductions)
+against the FP64 scheme, per field, after N steps of the blast."""
+import numpy as np
+import pytest
+import torch
+
+from fields import field_errors
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5  # relative L-inf per field, rho p u v (w)
+
+
+def _run(gpu, nb, steps, precision, ic_mode=0, radius_frac=0.0):
+    dim = len(nb)
+    C = 0.5 if dim == 2 else 0.4
+    k = gpu.constants(nb, C)
+    shape = tuple(n + 2 for n in nb)
+    a = [torch.zeros(shape + (2,), dtype=torch.float64, device="cuda"),
+         torch.zeros(shape + (dim,), dtype=torch.float64, device="cuda")]
+    b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+    dt0 = gpu.ic(a[0], a[1], k["C_min"], ic_mode=ic_mode, radius_frac=radius_frac)
+    m0 = gpu.total_mass(a[0])
+    run = gpu.Run(nb, k["D"], k["C_min"], dt0, 10.0, steps, precision)
+    run.advance(a, b, steps)
+    st = run.status()
+    assert st["rc"] == 0 and st["it"] == steps, st
+    cur = a if st["current"] == 0 else b
+    m1 = gpu.total_mass(cur[0])
+    out = (cur[0].cpu().numpy(), cur[1].cpu().numpy())
+    tt, td = run.trajectory(steps)
+    run.close()
+    return out, td, abs(m1 - m0) / m0
+
+
+@pytest.mark.parametrize("nb,steps", [((400, 300), 300), ((1000, 700), 200)])
+def test_fp32_flux_2d_vs_fp64(gpu, nb, steps):
+    ref, td_ref, dm_ref = _run(gpu, nb, steps, gpu.FP64)
+    got, td_got, dm_got = _run(gpu, nb, steps, gpu.FP32_FLUX)
+    errs = field_errors(got, ref)
+    print(f"[fp32-flux 2D {nb} x {steps}] " + ", ".join(f"{k}: Linf {v[0]:.2e} L2 {v[1]:.2e}" for k, v in errs.items())
+          + f"; Dt rel {np.max(np.abs(td_got / td_ref - 1)):.2e}; mass drift {dm_got:.1e} (fp64 {dm_ref:.1e})")
+    for k, (linf, _) in errs.items():
+        assert linf <= TOL, (k, linf)
+    assert dm_got < 1e-13
+
+
+@pytest.mark.parametrize("nb,steps", [((64, 64, 64), 100), ((96, 80, 72), 80)])
+def test_fp32_flux_3d_vs_fp64(gpu, nb, steps):
+    ref, td_ref, dm_ref = _run(gpu, nb, steps, gpu.FP64, ic_mode=1, radius_frac=0.1)
+    got, td_got, dm_got = _run(gpu, nb, steps, gpu.FP32_FLUX, ic_mode=1, radius_frac=0.1)
+    errs = field_errors(got, ref)
+    print(f"[fp32-flux 3D {nb} x {steps}] " + ", ".join(f"{k}: Linf {v[0]:.2e} L2 {v[1]:.2e}" for k, v in errs.items())
+          + f"; Dt rel {np.max(np.abs(td_got / td_ref - 1)):.2e}; mass drift {dm_got:.1e}")
+    for k, (linf, _) in errs.items():
+        assert linf <= TOL, (k, linf)
+    assert dm_got < 1e-13
