@@ -17,8 +17,10 @@
 // wave row is one contiguous 1 KiB stream per array.
 //
 // Dt: every lane folds the signal speed of its updated cells into a running max
-// (as an order key); waves reduce by shuffles, workgroups by one 64-bit atomic max,
-// and the last workgroup to finish (ticket counter) computes Dt = C_min / max_speed.
+// (as an order key); waves reduce by shuffles, and each workgroup does one 64-bit
+// atomic max into one of 64 slots (spread by workgroup index).  A one-wave finalize
+// kernel, launched after the step kernel on the same stream, folds the slots, computes
+// Dt = C_min / max_speed and clears the slots for the next step.
 #include <hip/hip_runtime.h>
 
 #include "../../include/hydro_hip.h"
