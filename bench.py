@@ -34,7 +34,10 @@ WORKLOADS = {
     "blast3d_512_fp32flux": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80, 3264),
 }
 ALIASES = {"2d": "blast2d_40000x20000_fp64", "3d": "blast3d_512_fp64", "2d-fp32flux": "blast2d_40000x20000_fp32flux",
-           "3d-fp32flux": "blast3d_512_fp32flux"}
+           "3d-fp32flux": "blast3d_512_fp32flux", "batched": "batched2d_256_fp32flux",
+           "batched-fp64": "batched2d_256_fp64"}
+# Batched workloads: 256 seeded 2D blasts (sides log-uniform in [64, 2048]), precision.
+BATCHED = {"batched2d_256_fp32flux": 3264, "batched2d_256_fp64": 64}
 
 
 def log(*a):
@@ -93,6 +96,130 @@ def load_traffic(workload: str):
     return None
 
 
+def _blast_bufs(torch, nb):
+    shape = tuple(n + 2 for n in nb)
+    a = [torch.zeros(shape + (2,), dtype=torch.float64, device="cuda"),
+         torch.zeros(shape + (len(nb),), dtype=torch.float64, device="cuda")]
+    return a, [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+
+
+def bench_batched(args, name: str) -> dict:
+    """256 independent 2D blasts of uneven sizes in one batch: autotuned tiling, steps
+    replayed from a hipGraph, K timed steps.  Beside it, the same blasts solved one at a
+    time (a sample of steps per blast): the device-resident single-solve loop, and the
+    reference loop shape (ghost fill, synchronous launcher, Dt read back per step)."""
+    import numpy as np
+    import torch
+
+    hydro = importlib.import_module("scalable-fpga-hydrodynamics_amd")
+    wl = importlib.import_module("scalable-fpga-hydrodynamics_amd.workloads")
+    precision = BATCHED[name]
+    blasts = wl.batched_blasts(256)
+    cells = sum(int(np.prod(p["nb"])) for p in blasts)
+    log(f"[bench] {name}: {len(blasts)} blasts, {cells / 1e6:.1f} M cells, "
+        f"{sum(wl.state_bytes(p['nb']) for p in blasts) / 1e9:.2f} GB of state")
+    bufs = [_blast_bufs(torch, p["nb"]) for p in blasts]
+    torch.cuda.synchronize()
+    batch = hydro.Batch(2, precision, len(blasts))
+    for p, (a, b) in zip(blasts, bufs):
+        batch.add_blast(a, b, p["C"], p["radius_frac"], p["p_in"], p["p_out"], p["rho_in"], p["rho_out"],
+                        max_t=1e30, max_it=10**9)
+    t0 = time.perf_counter()
+    tune_ms = batch.autotune(steps_per_trial=2, use_cache=False)
+    tuning = batch.tiling()
+    log(f"[bench] autotune {time.perf_counter() - t0:.2f} s: blocks/CU {tuning[0]}, tile rows {tuning[1]} "
+        f"({tune_ms:.3f} ms/step)")
+    graph_steps = 10
+    batch.set_graph(graph_steps)
+    W, K = args.warmup, args.steps
+    batch.advance(W)
+    batch.sync()
+    it0 = batch.status(0)["it"]
+    ext = torch.cuda.ExternalStream(batch.stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    w0 = time.perf_counter()
+    e0.record(ext)
+    batch.advance(K)
+    e1.record(ext)
+    batch.sync()
+    wall = time.perf_counter() - w0
+    ms = e0.elapsed_time(e1)
+    elapsed = max(wall, ms / 1e3)
+    gcups = cells * K / elapsed / 1e9
+    per_step = elapsed / K
+    flags = batch.done_flags(len(blasts))
+    st = [batch.status(s) for s in range(len(blasts))]
+    bad = [i for i, x in enumerate(st) if x["rc"] != 0 or x["it"] != it0 + K]
+    if any(flags) or bad:
+        raise RuntimeError(f"batched run: unexpected stop flags / status {flags.nonzero()} {bad}")
+    log(f"[bench] batched: {K} steps in {elapsed * 1e3:.1f} ms, {gcups:.2f} GCUPS")
+    batch.close()
+
+    # One solve at a time, same blasts, a sample of steps each (GCUPS is per-step).
+    ks = max(1, min(K, args.plain_steps))
+    k = hydro.constants
+    t_res = 0.0
+    t_sync = 0.0
+    n_sync = min(len(blasts), args.plain_sync_blasts)
+    for i, (p, (a, b)) in enumerate(zip(blasts, bufs)):
+        kk = k(p["nb"], p["C"])
+        dt0 = hydro.ic(a[0], a[1], kk["C_min"], ic_mode=1, radius_frac=p["radius_frac"], p_in=p["p_in"],
+                       p_out=p["p_out"], rho_in=p["rho_in"], rho_out=p["rho_out"])
+        run = hydro.Run(p["nb"], kk["D"], kk["C_min"], dt0, 1e30, ks, precision)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.advance(a, b, ks)
+        torch.cuda.synchronize()
+        t_res += time.perf_counter() - t0
+        run.close()
+        if i < n_sync:
+            # The reference loop shape (hydro2d/main.cxx:712-815): ghost fill, synchronous
+            # launcher, Dt read back and DtDx / DtDy recomputed on the host every step.
+            dt = hydro.ic(a[0], a[1], kk["C_min"], ic_mode=1, radius_frac=p["radius_frac"], p_in=p["p_in"],
+                          p_out=p["p_out"], rho_in=p["rho_in"], rho_out=p["rho_out"])
+            cur, nxt = a, b
+            dtn = torch.zeros(1, dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(ks):
+                hydro.ghost_fill2d(cur[0], cur[1])
+                hydro.launcher(cur[0], cur[1], nxt[0], nxt[1], dtn, kk["C_min"], kk["gamma"], kk["gmo"],
+                               kk["divgamma"], kk["K"], dt / kk["D"][0], dt / kk["D"][1])
+                dt = float(dtn.item())
+                cur, nxt = nxt, cur
+            t_sync += time.perf_counter() - t0
+    sync_cells = sum(int(np.prod(p["nb"])) for p in blasts[:n_sync])
+    plain_res = cells * ks / t_res / 1e9
+    plain_sync = sync_cells * ks / t_sync / 1e9 if n_sync else None
+    log(f"[bench] one solve at a time: resident {plain_res:.2f} GCUPS, synchronous launcher loop "
+        f"{plain_sync if plain_sync is None else round(plain_sync, 2)} GCUPS")
+    return {
+        "metric": "cell-updates/s (GCUPS) + % HBM roofline",
+        "value": round(gcups, 4),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(per_step * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if precision == 64 else "f64 state, f32 flux",
+        "data": "synthetic (256 seeded blasts, splitmix64 seed 0x5EED, IC generated on the device)",
+        "config": {"workload": name, "blasts": len(blasts), "cells": cells, "sides": "log-uniform [64, 2048]",
+                   "tiling": {"blocks_per_cu": tuning[0], "tile_rows": tuning[1]}, "graph_steps": graph_steps,
+                   "kernel": "batch2d (persistent, one launch per step) + batch finalize"},
+        "roofline": {"bound": "hbm", "achieved": round(64 * cells / per_step / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(64 * cells / per_step / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+        "plain_resident_gcups": round(plain_res, 4),
+        "plain_sync_launcher_gcups": round(plain_sync, 4) if plain_sync else None,
+        "speedup_vs_plain_resident": round(gcups / plain_res, 3),
+        "speedup_vs_plain_sync_launcher": round(gcups / plain_sync, 3) if plain_sync else None,
+        "plain_sample": f"{ks} steps per blast, all {len(blasts)} blasts (resident) / first {n_sync} (synchronous)",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +229,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=0, help="override the kernel's X rows per strip")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-reference", action="store_true", help="skip the achievable-bandwidth copy")
+    ap.add_argument("--plain-steps", type=int, default=20, help="steps per blast of the one-solve-at-a-time baselines")
+    ap.add_argument("--plain-sync-blasts", type=int, default=256)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -112,6 +241,14 @@ def main() -> int:
         return 2
 
     name = ALIASES.get(args.workload, args.workload)
+    if name in BATCHED:
+        import torch
+
+        if not torch.cuda.is_available():
+            log("no GPU visible")
+            return 1
+        print(json.dumps(bench_batched(args, name)), flush=True)
+        return 0
     if name not in WORKLOADS:
         log(f"unknown workload {args.workload}; choose from {sorted(WORKLOADS)}")
         return 2

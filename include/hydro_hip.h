@@ -172,6 +172,73 @@ int hydro_ic_f32(float* d_rhoE, float* d_mom, int dim, const size_t* nb, const s
                  float gamma, float* min_dt, void* stream);
 
 /* ---------------------------------------------------------------------------------
+ * Batched form: many independent solves of uneven sizes in one engine.
+ *
+ * No single counterpart in the reference, which runs one solve per process; each blast
+ * of a batch is one complete run of hydro2d/main.cxx:686-815 (hydro3d/main.cxx:742-876)
+ * with the reference's boundaries (X periodic, Y and Z reflective) and step arithmetic
+ * (hydro2d/kernel.cxx:132-319, hydro3d/kernel.cxx:145-407), bit-identical to a
+ * single-domain hydro_run of the same blast.  One persistent kernel per step advances
+ * every active blast, followed by one finalise launch that computes each blast's Dt,
+ * advances its t / iteration count and flips its current buffer; a blast stops on its
+ * own when !(t < max_t && it < max_it).  Steps may be replayed from a hipGraph.  The
+ * batch runs on its own non-blocking stream (hydro_batch_stream); buffers handed to it
+ * must be ready before the first advance (synchronise the producing stream).
+ * ------------------------------------------------------------------------------- */
+
+typedef struct hydro_batch hydro_batch;
+
+/* dim 2 or 3; precision HYDRO_FP64 or HYDRO_FP32_FLUX (FP64 state); capacity = maximum
+ * number of blasts held at once.  Uses the current device.  NULL on failure. */
+hydro_batch* hydro_batch_create(int dim, int precision, size_t capacity);
+void hydro_batch_destroy(hydro_batch* batch);
+void* hydro_batch_stream(hydro_batch* batch);
+
+/* Add a blast whose state (reference layout, interior sizes nb[dim]) is in rhoE_a/mom_a;
+ * rhoE_b/mom_b are its second buffer pair.  D = grid spacings, dt0 = initial Dt (as
+ * hydro_run_create).  Returns the slot (>= 0) or a negative error code. */
+int hydro_batch_add(hydro_batch* batch, const size_t* nb, const double* D, double C_min, double dt0, double max_t,
+                    size_t max_it, void* rhoE_a, void* mom_a, void* rhoE_b, void* mom_b);
+
+/* Add a blast and write its initial condition on the device (asynchronously, on the
+ * batch stream): hydro_ic's parameterised blast (ic_mode 1: radius = radius_frac *
+ * min(nb + 1), centre (nb + 1) / 2), spacings D = 1 / (nb + 1), C_min = C * min(D)
+ * (hydro2d/main.cxx:482-484, 595-607), initial Dt reduced on the device. */
+int hydro_batch_add_blast(hydro_batch* batch, const size_t* nb, double C, double radius_frac, double rho_in,
+                          double rho_out, double p_in, double p_out, double max_t, size_t max_it, void* rhoE_a,
+                          void* mom_a, void* rhoE_b, void* mom_b);
+
+/* Free a slot (waits for the batch stream; the buffers stay the caller's). */
+int hydro_batch_remove(hydro_batch* batch, int slot);
+
+/* Enqueue n_steps steps of every active blast (asynchronous). */
+int hydro_batch_advance(hydro_batch* batch, size_t n_steps);
+int hydro_batch_sync(hydro_batch* batch);
+
+/* Iteration, t, next Dt, stop flag and current buffer (0 = a, 1 = b) of one slot (waits
+ * for the batch stream).  Returns HYDRO_ERR_NONFINITE if its Dt became NaN or inf. */
+int hydro_batch_status(hydro_batch* batch, int slot, size_t* it, double* t, double* dt, int* done, int* current);
+
+/* Stop flags of slots [0, n) in one copy: 0 running, 1 done, 2 stopped on a
+ * non-finite Dt (waits for the batch stream). */
+int hydro_batch_done_flags(hydro_batch* batch, unsigned char* flags, size_t n);
+
+/* Tiling: workgroups per CU of the persistent grid (0 = default) and tile rows (0 =
+ * balanced: equal contiguous row ranges per worker; R > 0: R-row tiles round-robin).
+ * Results do not depend on the tiling. */
+int hydro_batch_set_tiling(hydro_batch* batch, int blocks_per_cu, int tile_rows);
+int hydro_batch_get_tiling(hydro_batch* batch, int* blocks_per_cu, int* tile_rows);
+
+/* Replay steps from a hipGraph of steps_per_graph steps (0 = direct launches). */
+int hydro_batch_set_graph(hydro_batch* batch, size_t steps_per_graph);
+
+/* Time every tiling candidate for steps_per_trial steps on the batch itself (they are
+ * ordinary steps of the solves) and keep the fastest; with use_cache, a batch of the
+ * same signature (dimension, precision, blast count, total cells) tuned earlier in the
+ * process reuses that result without trials (*best_ms_per_step = -1 then). */
+int hydro_batch_autotune(hydro_batch* batch, size_t steps_per_trial, int use_cache, double* best_ms_per_step);
+
+/* ---------------------------------------------------------------------------------
  * Diagnostics.
  * ------------------------------------------------------------------------------- */
 

@@ -86,6 +86,23 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_tune_get.argtypes = [C.c_char_p]
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_stream_copy.argtypes = [vp, vp, sz, vp]
+    L.hydro_batch_create.argtypes = [i32, i32, sz]
+    L.hydro_batch_create.restype = vp
+    L.hydro_batch_destroy.argtypes = [vp]
+    L.hydro_batch_destroy.restype = None
+    L.hydro_batch_stream.argtypes = [vp]
+    L.hydro_batch_stream.restype = vp
+    L.hydro_batch_add.argtypes = [vp, vp, vp, d, d, d, sz, vp, vp, vp, vp]
+    L.hydro_batch_add_blast.argtypes = [vp, vp, d, d, d, d, d, d, d, sz, vp, vp, vp, vp]
+    L.hydro_batch_remove.argtypes = [vp, i32]
+    L.hydro_batch_advance.argtypes = [vp, sz]
+    L.hydro_batch_sync.argtypes = [vp]
+    L.hydro_batch_status.argtypes = [vp, i32, vp, vp, vp, vp, vp]
+    L.hydro_batch_done_flags.argtypes = [vp, vp, sz]
+    L.hydro_batch_set_tiling.argtypes = [vp, i32, i32]
+    L.hydro_batch_get_tiling.argtypes = [vp, vp, vp]
+    L.hydro_batch_set_graph.argtypes = [vp, sz]
+    L.hydro_batch_autotune.argtypes = [vp, sz, i32, vp]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
 
@@ -277,6 +294,108 @@ class Run:
         if getattr(self, "_h", None):
             lib().hydro_run_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many independent blasts advanced together by one persistent kernel per step
+    (include/hydro_hip.h, hydro_batch_*).  Each blast is a full run of the reference step
+    loop with X periodic and Y/Z reflective boundaries; its buffers (a, b) are pairs
+    (rhoE, mom) of contiguous float64 tensors in the reference layout."""
+
+    def __init__(self, dim: int, precision: int = FP64, capacity: int = 256):
+        h = lib().hydro_batch_create(int(dim), int(precision), int(capacity))
+        if not h:
+            raise HydroError(f"hydro_batch_create failed: {last_error()}")
+        self._h = C.c_void_p(h)
+        self.dim = dim
+        self.capacity = capacity
+        self._bufs = {}
+
+    @property
+    def stream(self) -> int:
+        return lib().hydro_batch_stream(self._h) or 0
+
+    def add(self, a, b, D, C_min, dt0, max_t, max_it) -> int:
+        nb = tuple(n - 2 for n in a[0].shape[: self.dim])
+        slot = lib().hydro_batch_add(self._h, _sizes(nb), (C.c_double * 3)(*[float(x) for x in D] + [0.0] * (3 - len(D))),
+                                     float(C_min), float(dt0), float(max_t), int(max_it), _p(a[0]), _p(a[1]), _p(b[0]),
+                                     _p(b[1]))
+        if slot < 0:
+            raise HydroError(f"hydro_batch_add failed ({slot}): {last_error()}")
+        self._bufs[slot] = (a, b)
+        return slot
+
+    def add_blast(self, a, b, C, radius_frac, p_in, p_out=0.1, rho_in=1.0, rho_out=1.2, max_t=1e30,
+                  max_it=100) -> int:
+        nb = tuple(n - 2 for n in a[0].shape[: self.dim])
+        slot = lib().hydro_batch_add_blast(self._h, _sizes(nb), float(C), float(radius_frac), float(rho_in),
+                                           float(rho_out), float(p_in), float(p_out), float(max_t), int(max_it),
+                                           _p(a[0]), _p(a[1]), _p(b[0]), _p(b[1]))
+        if slot < 0:
+            raise HydroError(f"hydro_batch_add_blast failed ({slot}): {last_error()}")
+        self._bufs[slot] = (a, b)
+        return slot
+
+    def remove(self, slot: int) -> None:
+        _check(lib().hydro_batch_remove(self._h, int(slot)), "hydro_batch_remove")
+        self._bufs.pop(slot, None)
+
+    def advance(self, n_steps: int) -> None:
+        _check(lib().hydro_batch_advance(self._h, int(n_steps)), "hydro_batch_advance")
+
+    def sync(self) -> None:
+        _check(lib().hydro_batch_sync(self._h), "hydro_batch_sync")
+
+    def status(self, slot: int) -> dict:
+        it, t, dt, done, cur = C.c_size_t(), C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        rc = lib().hydro_batch_status(self._h, int(slot), C.byref(it), C.byref(t), C.byref(dt), C.byref(done),
+                                      C.byref(cur))
+        st = {"it": it.value, "t": t.value, "dt": dt.value, "done": bool(done.value), "current": cur.value, "rc": rc}
+        if rc != 0:
+            st["error"] = last_error()
+        return st
+
+    def current(self, slot: int):
+        """The (rhoE, mom) pair holding the slot's current state."""
+        a, b = self._bufs[slot]
+        return a if self.status(slot)["current"] == 0 else b
+
+    def done_flags(self, n: int | None = None):
+        import numpy as np
+
+        n = self.capacity if n is None else n
+        out = np.zeros(n, np.uint8)
+        _check(lib().hydro_batch_done_flags(self._h, out.ctypes.data_as(C.c_void_p), n), "hydro_batch_done_flags")
+        return out
+
+    def set_tiling(self, blocks_per_cu: int = 0, tile_rows: int = 0) -> None:
+        _check(lib().hydro_batch_set_tiling(self._h, int(blocks_per_cu), int(tile_rows)), "hydro_batch_set_tiling")
+
+    def tiling(self):
+        bpc, rows = C.c_int(), C.c_int()
+        _check(lib().hydro_batch_get_tiling(self._h, C.byref(bpc), C.byref(rows)), "hydro_batch_get_tiling")
+        return bpc.value, rows.value
+
+    def set_graph(self, steps_per_graph: int) -> None:
+        _check(lib().hydro_batch_set_graph(self._h, int(steps_per_graph)), "hydro_batch_set_graph")
+
+    def autotune(self, steps_per_trial: int = 2, use_cache: bool = True) -> float:
+        ms = C.c_double()
+        _check(lib().hydro_batch_autotune(self._h, int(steps_per_trial), 1 if use_cache else 0, C.byref(ms)),
+               "hydro_batch_autotune")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hydro_batch_destroy(self._h)
+            self._h = None
+            self._bufs = {}
 
     def __del__(self):
         try:

@@ -26,6 +26,7 @@
 #include "../../include/hydro_hip.h"
 #include "hydro_device.hpp"
 #include "hydro_kernels.hpp"
+#include "hydro_march2d.hpp"
 #include "hydro_runtime.hpp"
 
 namespace hydro {
@@ -36,50 +37,6 @@ __device__ StepReduce g_red2d[2];
 
 namespace {
 
-constexpr int kOwned = kWave - 2;  // owned columns per wave window
-constexpr int kWavesPerBlock = 4;
-#ifndef HYDRO_2D_MIN_BLOCKS
-#define HYDRO_2D_MIN_BLOCKS 5
-#endif
-// Rows in flight ahead of the row being converted (register prefetch ring depth).
-#ifndef HYDRO_2D_PREFETCH
-#define HYDRO_2D_PREFETCH 3
-#endif
-constexpr int kPF = HYDRO_2D_PREFETCH;
-// Non-temporal (streaming) stores of the new state.
-#ifndef HYDRO_2D_NT_STORE
-#define HYDRO_2D_NT_STORE 0
-#endif
-constexpr int kBlock = kWave * kWavesPerBlock;
-
-template <typename T>
-struct Vec2;
-template <>
-struct Vec2<double> {
-    using type = double2;
-};
-template <>
-struct Vec2<float> {
-    using type = float2;
-};
-
-template <typename T>
-__device__ __forceinline__ void store_pair(typename Vec2<T>::type* p, T a, T b)
-{
-    if constexpr (HYDRO_2D_NT_STORE) {
-        typedef T v2t __attribute__((ext_vector_type(2)));
-        v2t v;
-        v.x = a;
-        v.y = b;
-        __builtin_nontemporal_store(v, reinterpret_cast<v2t*>(p));
-    } else {
-        typename Vec2<T>::type v;
-        v.x = a;
-        v.y = b;
-        *p = v;
-    }
-}
-
 struct Geom2 {
     int nb_x, nb_y;
     long long ys;  // nb_y + 2
@@ -87,31 +44,6 @@ struct Geom2 {
     int nwin;      // column windows
     int nwb;       // workgroups across Y (nwin / kWavesPerBlock, rounded up)
 };
-
-// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2): give each
-// XCD a contiguous range of logical blocks so that neighbouring column windows, which
-// share halo columns, run on the same L2.
-__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
-{
-    constexpr unsigned X = 8;
-    const unsigned q = n / X, r = n % X;
-    const unsigned xcd = b % X, k = b / X;
-    return xcd * q + (xcd < r ? xcd : r) + k;
-}
-
-template <typename T>
-struct Consts {
-    T C_min, gamma, gmo, divgamma, K;
-};
-
-template <typename T>
-__device__ __forceinline__ void load_cell(const typename Vec2<T>::type* __restrict__ re,
-                                          const typename Vec2<T>::type* __restrict__ mo, long long n,
-                                          typename Vec2<T>::type& a, typename Vec2<T>::type& b)
-{
-    a = re[n];
-    b = mo[n];
-}
 
 // TS: state and update precision; TF, M: precision and math policy of the primitives,
 // fluxes and signal speeds.  <double, double, IeeeMath> is the reference FP64 scheme,
@@ -146,119 +78,16 @@ __global__ void __launch_bounds__(kBlock, HYDRO_2D_MIN_BLOCKS)
     const int win = wb * kWavesPerBlock + wv;
     const int j = win * kOwned + lane;
     const bool owned = (win < g.nwin) && lane >= 1 && lane <= kOwned && j <= g.nb_y;
-    // Column actually loaded by this lane.  With reflective walls the two ghost columns
-    // load their interior neighbour and present its mirror (v negated) to the Y face,
-    // which is the reference's wall state (kernel.cxx:214-229, 234-250): lane j = 0
-    // evaluates flux(mirror(c1), c1) for column 1, and lane j = nb_y + 1 hands
-    // mirror(c_nb_y) to column nb_y.  Negation is exact, so prims(mirror) == mirror(prims).
-    // Periodic Y reads the ghost columns filled by the ghost fill.  Lanes past nb_y + 1
-    // are clamped in bounds and never owned.
     int jl;
-    unsigned int ymask = 0u;
-    if constexpr (WALL_Y) {
-        jl = j < 1 ? 1 : (j > g.nb_y ? g.nb_y : j);
-        ymask = (j == 0 || j == g.nb_y + 1) ? kSignMask<TF> : 0u;
-    } else {
-        jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
-    }
+    unsigned int ymask;
+    window_lane2d<TF, WALL_Y, false>(j, g.nb_y, jl, ymask);
     const int i0 = 1 + strip * g.rows;
     const int i1 = min(i0 + g.rows - 1, g.nb_x);
 
     uint64_t key = 0;
-    if (win < g.nwin) {
-        // Row i0-1 and i0 primitives; X face (i0-1, i0).  Rows i0+1 .. i0+kPF are
-        // loaded ahead into a register ring (qr/qm), so that kPF rows per wave are in
-        // flight while a row is computed.
-        V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
-        load_cell<TS>(R2, M2, static_cast<long long>(i0 - 1) * g.ys + jl, ra, ma);
-        load_cell<TS>(R2, M2, static_cast<long long>(i0) * g.ys + jl, rc, mc);
-#pragma unroll
-        for (int q = 0; q < kPF; ++q)
-            load_cell<TS>(R2, M2, static_cast<long long>(min(i0 + 1 + q, i1 + 1)) * g.ys + jl, qr[q], qm[q]);
-        Prim2<TF> Pm, Pc;  // X frame: un = u, ut = v
-        Pm.r = static_cast<TF>(ra.x);
-        cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma,
-                             Pm.un, Pm.ut, Pm.p, Pm.a);
-        Pc.r = static_cast<TF>(rc.x);
-        cons_to_prim2<TF, M>(Pc.r, static_cast<TF>(mc.x), static_cast<TF>(mc.y), static_cast<TF>(rc.y), gmo, gamma,
-                             Pc.un, Pc.ut, Pc.p, Pc.a);
-        TF fx1_0, fx1_1, fx1_2, fx1_3;
-        face_flux2<TF, M>(Pm, Pc, K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
-
-        for (int i = i0; i <= i1; ++i) {
-            // Prefetch row i+1+kPF (clamped to the last row this strip reads).
-            const int ipf = min(i + 1 + kPF, i1 + 1);
-            load_cell<TS>(R2, M2, static_cast<long long>(ipf) * g.ys + jl, qr[kPF], qm[kPF]);
-            const V2 rn = qr[0], mn = qm[0];  // row i+1
-
-            // Y face (j, j+1) of row i in the (rho, v, u, p) frame (kernel.cxx:213-251),
-            // evaluated by lane j and handed to lane j+1.
-            Prim2<TF> Sy, Ny;
-            Sy.r = Pc.r;
-            Sy.un = xor_sign(Pc.ut, ymask);
-            Sy.ut = Pc.un;
-            Sy.p = Pc.p;
-            Sy.a = Pc.a;
-            Ny.r = from_next_lane(Sy.r);
-            Ny.un = from_next_lane(Sy.un);
-            Ny.ut = from_next_lane(Sy.ut);
-            Ny.p = from_next_lane(Sy.p);
-            Ny.a = from_next_lane(Sy.a);
-            TF fyR0, fyR1, fyR2, fyR3;
-            face_flux2<TF, M>(Sy, Ny, K, divgamma, fyR0, fyR1, fyR2, fyR3);
-            const TF fyL0 = from_prev_lane(fyR0);
-            const TF fyL1 = from_prev_lane(fyR1);
-            const TF fyL2 = from_prev_lane(fyR2);
-            const TF fyL3 = from_prev_lane(fyR3);
-
-            // Row i+1 primitives and the X face (i, i+1).
-            Prim2<TF> Pn;
-            Pn.r = static_cast<TF>(rn.x);
-            cons_to_prim2<TF, M>(Pn.r, static_cast<TF>(mn.x), static_cast<TF>(mn.y), static_cast<TF>(rn.y), gmo, gamma,
-                                 Pn.un, Pn.ut, Pn.p, Pn.a);
-            TF fx2_0, fx2_1, fx2_2, fx2_3;
-            face_flux2<TF, M>(Pc, Pn, K, divgamma, fx2_0, fx2_1, fx2_2, fx2_3);
-
-            // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the
-            // (rho, v, u, E) frame, so rho*u takes component 2 and rho*v component 1.
-            // Increments and update are in the state precision TS.
-            const TS x_r = DtDx * (static_cast<TS>(fx1_0) - static_cast<TS>(fx2_0));
-            const TS x_u = DtDx * (static_cast<TS>(fx1_1) - static_cast<TS>(fx2_1));
-            const TS x_v = DtDx * (static_cast<TS>(fx1_2) - static_cast<TS>(fx2_2));
-            const TS x_E = DtDx * (static_cast<TS>(fx1_3) - static_cast<TS>(fx2_3));
-            const TS y_r = DtDy * (static_cast<TS>(fyL0) - static_cast<TS>(fyR0));
-            const TS y_u = DtDy * (static_cast<TS>(fyL2) - static_cast<TS>(fyR2));
-            const TS y_v = DtDy * (static_cast<TS>(fyL1) - static_cast<TS>(fyR1));
-            const TS y_E = DtDy * (static_cast<TS>(fyL3) - static_cast<TS>(fyR3));
-            const TS nr = rc.x + x_r + y_r;
-            const TS nu = mc.x + x_u + y_u;
-            const TS nv = mc.y + x_v + y_v;
-            const TS nE = rc.y + x_E + y_E;
-            const TF sp = signal_speed2<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
-                                               static_cast<TF>(nE), gmo, gamma);
-            const uint64_t k_new = speed_key(static_cast<double>(sp));
-            key = owned ? umax64(key, k_new) : key;
-            if (owned) {
-                const long long n = static_cast<long long>(i) * g.ys + j;
-                store_pair<TS>(RN2 + n, nr, nE);
-                store_pair<TS>(MN2 + n, nu, nv);
-            }
-
-            // Advance the march.
-            Pc = Pn;
-            rc = rn;
-            mc = mn;
-#pragma unroll
-            for (int q = 0; q < kPF; ++q) {
-                qr[q] = qr[q + 1];
-                qm[q] = qm[q + 1];
-            }
-            fx1_0 = fx2_0;
-            fx1_1 = fx2_1;
-            fx1_2 = fx2_2;
-            fx1_3 = fx2_3;
-        }
-    }
+    if (win < g.nwin)
+        key = march2d<TS, TF, M, false>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned, gamma, gmo, divgamma,
+                                        K, DtDx, DtDy, key);
 
     block_max_to_slot(red, key);
 }

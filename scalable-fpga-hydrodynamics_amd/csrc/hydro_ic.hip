@@ -76,6 +76,65 @@ using namespace hydro;
 
 namespace {
 
+// Host-side parameters of the blast: centre, radius, E = p / (gamma - 1) of the two
+// states and their Dt (hydro2d/main.cxx:288-333).
+template <typename T>
+struct IcSetup {
+    double c[3];
+    double radius;
+    T E_in, E_out, dt_in, dt_out;
+};
+
+template <typename T>
+IcSetup<T> ic_setup(int dim, const size_t* NB, T rho_in, T rho_out, T p_in, T p_out, int ic_mode, double radius_frac,
+                    T C_min, T gamma)
+{
+    IcSetup<T> q;
+    const T gmo = gamma - T(1);
+    q.c[0] = q.c[1] = q.c[2] = 0;
+    for (int d = 0; d < dim; ++d) q.c[d] = static_cast<double>(NB[d] + 1) / 2.0;
+    if (ic_mode == 0) {
+        double m = q.c[0];  // std::min({ centres }) / 3, hydro2d/main.cxx:292
+        for (int d = 1; d < dim; ++d)
+            if (q.c[d] < m) m = q.c[d];
+        q.radius = m / 3;
+    } else {
+        size_t n = NB[0];
+        for (int d = 1; d < dim; ++d)
+            if (NB[d] < n) n = NB[d];
+        q.radius = radius_frac * static_cast<double>(n + 1);
+    }
+    // E = p / (gamma - 1) (hydro2d/main.cxx:306) and the per-state Dt of :318-333.
+    q.E_in = p_in / gmo;
+    q.E_out = p_out / gmo;
+    auto dt_of = [&](T rho, T E) {
+        const T inv = T(1) / rho;
+        const T u = T(0) * inv;
+        const T tmp = u * u + u * u;
+        const T e = E * inv - T(0.5) * tmp;
+        const T p = gmo * rho * e;
+        const T cs = static_cast<T>(sizeof(T) == 8 ? sqrt(static_cast<double>(gamma * p * inv))
+                                                   : sqrtf(static_cast<float>(gamma * p * inv)));
+        const T un = static_cast<T>(sizeof(T) == 8 ? sqrt(static_cast<double>(tmp)) : sqrtf(static_cast<float>(tmp)));
+        return C_min / (cs + un);
+    };
+    q.dt_in = dt_of(rho_in, q.E_in);
+    q.dt_out = dt_of(rho_out, q.E_out);
+    return q;
+}
+
+template <typename T>
+void ic_launch(T* d_rhoE, T* d_mom, int dim, const size_t* nb, const size_t* base, const IcSetup<T>& q, T rho_in,
+               T rho_out, unsigned long long* d_flags, hipStream_t s)
+{
+    const size_t bz = (dim == 3 && base) ? base[2] : 0, nz = dim == 3 ? nb[2] : 0;
+    hipLaunchKernelGGL((ic_kernel<T>), dim3(2048), dim3(256), 0, s, d_rhoE, d_mom, dim, static_cast<long long>(nb[0]),
+                       static_cast<long long>(nb[1]), static_cast<long long>(nz),
+                       static_cast<long long>(base ? base[0] : 0), static_cast<long long>(base ? base[1] : 0),
+                       static_cast<long long>(bz), q.c[0], q.c[1], q.c[2], q.radius, rho_in, rho_out, q.E_in, q.E_out,
+                       q.dt_in, q.dt_out, d_flags);
+}
+
 template <typename T>
 int ic_impl(T* d_rhoE, T* d_mom, int dim, const size_t* nb, const size_t* base, const size_t* NB, T rho_in, T rho_out,
             T p_in, T p_out, int ic_mode, double radius_frac, T C_min, T gamma, T* min_dt, void* stream)
@@ -91,46 +150,13 @@ int ic_impl(T* d_rhoE, T* d_mom, int dim, const size_t* nb, const size_t* base, 
             return HYDRO_ERR_ARG;
         }
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const T gmo = gamma - T(1);
-    double c[3] = { 0, 0, 0 };
-    for (int d = 0; d < dim; ++d) c[d] = static_cast<double>(NB[d] + 1) / 2.0;
-    double radius;
-    if (ic_mode == 0) {
-        double m = c[0];  // std::min({ centres }) / 3, hydro2d/main.cxx:292
-        for (int d = 1; d < dim; ++d)
-            if (c[d] < m) m = c[d];
-        radius = m / 3;
-    } else {
-        size_t n = NB[0];
-        for (int d = 1; d < dim; ++d)
-            if (NB[d] < n) n = NB[d];
-        radius = radius_frac * static_cast<double>(n + 1);
-    }
-    // E = p / (gamma - 1) (hydro2d/main.cxx:306) and the per-state Dt of :318-333.
-    const T E_in = p_in / gmo, E_out = p_out / gmo;
-    auto dt_of = [&](T rho, T E) {
-        const T inv = T(1) / rho;
-        const T u = T(0) * inv;
-        const T tmp = u * u + u * u;
-        const T e = E * inv - T(0.5) * tmp;
-        const T p = gmo * rho * e;
-        const T cs = static_cast<T>(sizeof(T) == 8 ? sqrt(static_cast<double>(gamma * p * inv))
-                                                   : sqrtf(static_cast<float>(gamma * p * inv)));
-        const T un = static_cast<T>(sizeof(T) == 8 ? sqrt(static_cast<double>(tmp)) : sqrtf(static_cast<float>(tmp)));
-        return C_min / (cs + un);
-    };
-    const T dt_in = dt_of(rho_in, E_in), dt_out = dt_of(rho_out, E_out);
+    const IcSetup<T> q = ic_setup<T>(dim, NB, rho_in, rho_out, p_in, p_out, ic_mode, radius_frac, C_min, gamma);
 
     unsigned long long* d_flags = nullptr;
     HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&d_flags), sizeof(unsigned long long)), HYDRO_ERR_HIP);
     hipError_t e = hipMemsetAsync(d_flags, 0, sizeof(unsigned long long), s);
-    const size_t bz = (dim == 3 && base) ? base[2] : 0, nz = dim == 3 ? nb[2] : 0;
     if (e == hipSuccess) {
-        hipLaunchKernelGGL((ic_kernel<T>), dim3(2048), dim3(256), 0, s, d_rhoE, d_mom, dim,
-                           static_cast<long long>(nb[0]), static_cast<long long>(nb[1]), static_cast<long long>(nz),
-                           static_cast<long long>(base ? base[0] : 0), static_cast<long long>(base ? base[1] : 0),
-                           static_cast<long long>(base ? bz : 0), c[0], c[1], c[2], radius, rho_in, rho_out, E_in,
-                           E_out, dt_in, dt_out, d_flags);
+        ic_launch<T>(d_rhoE, d_mom, dim, nb, base, q, rho_in, rho_out, d_flags, s);
         e = hipGetLastError();
     }
     unsigned long long flags = 0;
@@ -143,13 +169,34 @@ int ic_impl(T* d_rhoE, T* d_mom, int dim, const size_t* nb, const size_t* base, 
     }
     // min_Dt starts at 1e20 and takes std::min over cells (hydro2d/main.cxx:287, 335).
     T m = static_cast<T>(1e20);
-    if (flags & 1u) m = (dt_in < m) ? dt_in : m;
-    if (flags & 2u) m = (dt_out < m) ? dt_out : m;
+    if (flags & 1u) m = (q.dt_in < m) ? q.dt_in : m;
+    if (flags & 2u) m = (q.dt_out < m) ? q.dt_out : m;
     if (min_dt) *min_dt = m;
     return 0;
 }
 
 }  // namespace
+
+namespace hydro {
+
+int ic_blast_async_f64(double* d_rhoE, double* d_mom, int dim, const size_t* nb, double rho_in, double rho_out,
+                       double p_in, double p_out, int ic_mode, double radius_frac, double C_min, double gamma,
+                       unsigned long long* d_flags, double* dt_in, double* dt_out, hipStream_t s)
+{
+    const IcSetup<double> q =
+        ic_setup<double>(dim, nb, rho_in, rho_out, p_in, p_out, ic_mode, radius_frac, C_min, gamma);
+    ic_launch<double>(d_rhoE, d_mom, dim, nb, nullptr, q, rho_in, rho_out, d_flags, s);
+    *dt_in = q.dt_in;
+    *dt_out = q.dt_out;
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("hydro IC: %s", hipGetErrorString(e));
+        return HYDRO_ERR_HIP;
+    }
+    return 0;
+}
+
+}  // namespace hydro
 
 extern "C" int hydro_ic(double* d_rhoE, double* d_mom, int dim, const size_t* nb, const size_t* base, const size_t* NB,
                         double rho_in, double rho_out, double p_in, double p_out, int ic_mode, double radius_frac,

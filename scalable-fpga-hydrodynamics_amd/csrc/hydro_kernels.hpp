@@ -23,6 +23,13 @@ int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int b
 int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
                 int precision, const StepControl* ctl, hipStream_t s);
 
+// Blast IC (hydro_ic's ic_mode 0/1) written asynchronously on `s`, origin at cell 1;
+// the occurring states are or-ed into *d_flags (bit 0 inside, bit 1 outside), which the
+// caller clears first, and the two states' Dt are returned on the host.
+int ic_blast_async_f64(double* d_rhoE, double* d_mom, int dim, const size_t* nb, double rho_in, double rho_out,
+                       double p_in, double p_out, int ic_mode, double radius_frac, double C_min, double gamma,
+                       unsigned long long* d_flags, double* dt_in, double* dt_out, hipStream_t s);
+
 void set_rows_2d(int rows);
 void set_tile_3d(int rows);
 int get_rows_2d();

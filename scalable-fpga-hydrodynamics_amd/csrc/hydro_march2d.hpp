@@ -1,0 +1,232 @@
+// The 2D strip march shared by the single-domain step kernel and the batched kernel.
+//
+// One wave (64 lanes, lane l <-> Y column j) marches along X over rows i0..i1 of its
+// column window, reading rows i0-1 .. i1+1:
+//   * the X face between rows i and i+1 is computed once and carried in registers to
+//     the next row (fx2 of row i is fx1 of row i+1; hydro2d/kernel.cxx:186-210 computes
+//     it twice with identical arguments);
+//   * the Y face (j, j+1) is computed once by lane j and handed to lane j+1 by a
+//     cross-lane shuffle (hydro2d/kernel.cxx:213-251);
+//   * primitives and the impedance sqrt(gamma*p*rho) are computed once per cell;
+//   * kPF rows of loads are kept in flight ahead of the row being converted.
+// The arithmetic of every cell is the reference's (hydro2d/kernel.cxx:132-319) in its
+// operation order, so results do not depend on how rows and windows are split.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hydro_device.hpp"
+
+namespace hydro {
+
+constexpr int kOwned = kWave - 2;  // owned columns per wave window
+constexpr int kWavesPerBlock = 4;
+#ifndef HYDRO_2D_MIN_BLOCKS
+#define HYDRO_2D_MIN_BLOCKS 5
+#endif
+// Rows in flight ahead of the row being converted (register prefetch ring depth).
+#ifndef HYDRO_2D_PREFETCH
+#define HYDRO_2D_PREFETCH 3
+#endif
+constexpr int kPF = HYDRO_2D_PREFETCH;
+// Non-temporal (streaming) stores of the new state.
+#ifndef HYDRO_2D_NT_STORE
+#define HYDRO_2D_NT_STORE 0
+#endif
+constexpr int kBlock = kWave * kWavesPerBlock;
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+
+template <typename T>
+__device__ __forceinline__ void store_pair(typename Vec2<T>::type* p, T a, T b)
+{
+    if constexpr (HYDRO_2D_NT_STORE) {
+        typedef T v2t __attribute__((ext_vector_type(2)));
+        v2t v;
+        v.x = a;
+        v.y = b;
+        __builtin_nontemporal_store(v, reinterpret_cast<v2t*>(p));
+    } else {
+        typename Vec2<T>::type v;
+        v.x = a;
+        v.y = b;
+        *p = v;
+    }
+}
+
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2): give each
+// XCD a contiguous range of logical blocks so that neighbouring column windows, which
+// share halo columns, run on the same L2.
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned n)
+{
+    constexpr unsigned X = 8;
+    const unsigned q = n / X, r = n % X;
+    const unsigned xcd = b % X, k = b / X;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+template <typename T>
+struct Consts {
+    T C_min, gamma, gmo, divgamma, K;
+};
+
+template <typename T>
+__device__ __forceinline__ void load_cell(const typename Vec2<T>::type* __restrict__ re,
+                                          const typename Vec2<T>::type* __restrict__ mo, long long n,
+                                          typename Vec2<T>::type& a, typename Vec2<T>::type& b)
+{
+    a = re[n];
+    b = mo[n];
+}
+
+// Row actually read for row r.  WRAP_X: rows 0 and nb_x + 1 are read from rows nb_x and
+// 1 (the periodic X ghost planes, hydro2d/main.cxx:349-378, without materialising them);
+// otherwise the ghost planes are read as filled by the ghost fill.
+template <bool WRAP_X>
+__device__ __forceinline__ long long row_of(int r, int nb_x)
+{
+    if constexpr (WRAP_X) return r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r);
+    else return r;
+}
+
+// Per-lane column set-up for a window whose lane 0 is column j = j_first (Y ghost
+// columns included).  With reflective Y walls the two ghost columns load their interior
+// neighbour and present its mirror (v negated) to the Y face, which is the reference's
+// wall state (hydro2d/kernel.cxx:214-229, 234-250): lane j = 0 evaluates
+// flux(mirror(c1), c1) for column 1 and lane j = nb_y + 1 hands mirror(c_nb_y) to column
+// nb_y.  Negation is exact, so prims(mirror) == mirror(prims).  Periodic Y reads the
+// ghost columns (filled by the ghost fill, or with WRAP_Y the opposite interior column).
+// Lanes past nb_y + 1 are clamped in bounds and never owned.
+template <typename TF, bool WALL_Y, bool WRAP_Y>
+__device__ __forceinline__ void window_lane2d(int j, int nb_y, int& jl, unsigned int& ymask)
+{
+    ymask = 0u;
+    if constexpr (WALL_Y) {
+        jl = j < 1 ? 1 : (j > nb_y ? nb_y : j);
+        ymask = (j == 0 || j == nb_y + 1) ? kSignMask<TF> : 0u;
+    } else if constexpr (WRAP_Y) {
+        jl = j == 0 ? nb_y : (j >= nb_y + 1 ? 1 : j);
+    } else {
+        jl = j <= nb_y + 1 ? j : nb_y + 1;
+    }
+}
+
+// March rows i0..i1 (i0 <= i1) of one window.  Lane columns: j (global, for stores and
+// ownership) and jl (the column loaded).  Returns `key` folded with the order keys of
+// the new signal speeds of the owned cells.
+template <typename TS, typename TF, typename M, bool WRAP_X>
+__device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __restrict__ R2,
+                                            const typename Vec2<TS>::type* __restrict__ M2,
+                                            typename Vec2<TS>::type* __restrict__ RN2,
+                                            typename Vec2<TS>::type* __restrict__ MN2, long long ys, int nb_x, int i0,
+                                            int i1, int j, int jl, unsigned int ymask, bool owned, TF gamma, TF gmo,
+                                            TF divgamma, TF K, TS DtDx, TS DtDy, uint64_t key)
+{
+    using V2 = typename Vec2<TS>::type;
+    // Rows i0-1 and i0; X face (i0-1, i0).  Rows i0+1 .. i0+kPF are loaded ahead into
+    // a register ring (qr/qm), so that kPF rows per wave are in flight while a row is
+    // computed.
+    V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
+    load_cell<TS>(R2, M2, row_of<WRAP_X>(i0 - 1, nb_x) * ys + jl, ra, ma);
+    load_cell<TS>(R2, M2, row_of<WRAP_X>(i0, nb_x) * ys + jl, rc, mc);
+#pragma unroll
+    for (int q = 0; q < kPF; ++q)
+        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i0 + 1 + q, i1 + 1), nb_x) * ys + jl, qr[q], qm[q]);
+    Prim2<TF> Pm, Pc;  // X frame: un = u, ut = v
+    Pm.r = static_cast<TF>(ra.x);
+    cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma, Pm.un,
+                         Pm.ut, Pm.p, Pm.a);
+    Pc.r = static_cast<TF>(rc.x);
+    cons_to_prim2<TF, M>(Pc.r, static_cast<TF>(mc.x), static_cast<TF>(mc.y), static_cast<TF>(rc.y), gmo, gamma, Pc.un,
+                         Pc.ut, Pc.p, Pc.a);
+    TF fx1_0, fx1_1, fx1_2, fx1_3;
+    face_flux2<TF, M>(Pm, Pc, K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
+
+    for (int i = i0; i <= i1; ++i) {
+        // Prefetch row i+1+kPF (clamped to the last row this strip reads).
+        const int ipf = min(i + 1 + kPF, i1 + 1);
+        load_cell<TS>(R2, M2, row_of<WRAP_X>(ipf, nb_x) * ys + jl, qr[kPF], qm[kPF]);
+        const V2 rn = qr[0], mn = qm[0];  // row i+1
+
+        // Y face (j, j+1) of row i in the (rho, v, u, p) frame (kernel.cxx:213-251),
+        // evaluated by lane j and handed to lane j+1.
+        Prim2<TF> Sy, Ny;
+        Sy.r = Pc.r;
+        Sy.un = xor_sign(Pc.ut, ymask);
+        Sy.ut = Pc.un;
+        Sy.p = Pc.p;
+        Sy.a = Pc.a;
+        Ny.r = from_next_lane(Sy.r);
+        Ny.un = from_next_lane(Sy.un);
+        Ny.ut = from_next_lane(Sy.ut);
+        Ny.p = from_next_lane(Sy.p);
+        Ny.a = from_next_lane(Sy.a);
+        TF fyR0, fyR1, fyR2, fyR3;
+        face_flux2<TF, M>(Sy, Ny, K, divgamma, fyR0, fyR1, fyR2, fyR3);
+        const TF fyL0 = from_prev_lane(fyR0);
+        const TF fyL1 = from_prev_lane(fyR1);
+        const TF fyL2 = from_prev_lane(fyR2);
+        const TF fyL3 = from_prev_lane(fyR3);
+
+        // Row i+1 primitives and the X face (i, i+1).
+        Prim2<TF> Pn;
+        Pn.r = static_cast<TF>(rn.x);
+        cons_to_prim2<TF, M>(Pn.r, static_cast<TF>(mn.x), static_cast<TF>(mn.y), static_cast<TF>(rn.y), gmo, gamma,
+                             Pn.un, Pn.ut, Pn.p, Pn.a);
+        TF fx2_0, fx2_1, fx2_2, fx2_3;
+        face_flux2<TF, M>(Pc, Pn, K, divgamma, fx2_0, fx2_1, fx2_2, fx2_3);
+
+        // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the
+        // (rho, v, u, E) frame, so rho*u takes component 2 and rho*v component 1.
+        // Increments and update are in the state precision TS.
+        const TS x_r = DtDx * (static_cast<TS>(fx1_0) - static_cast<TS>(fx2_0));
+        const TS x_u = DtDx * (static_cast<TS>(fx1_1) - static_cast<TS>(fx2_1));
+        const TS x_v = DtDx * (static_cast<TS>(fx1_2) - static_cast<TS>(fx2_2));
+        const TS x_E = DtDx * (static_cast<TS>(fx1_3) - static_cast<TS>(fx2_3));
+        const TS y_r = DtDy * (static_cast<TS>(fyL0) - static_cast<TS>(fyR0));
+        const TS y_u = DtDy * (static_cast<TS>(fyL2) - static_cast<TS>(fyR2));
+        const TS y_v = DtDy * (static_cast<TS>(fyL1) - static_cast<TS>(fyR1));
+        const TS y_E = DtDy * (static_cast<TS>(fyL3) - static_cast<TS>(fyR3));
+        const TS nr = rc.x + x_r + y_r;
+        const TS nu = mc.x + x_u + y_u;
+        const TS nv = mc.y + x_v + y_v;
+        const TS nE = rc.y + x_E + y_E;
+        const TF sp = signal_speed2<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
+                                           static_cast<TF>(nE), gmo, gamma);
+        const uint64_t k_new = speed_key(static_cast<double>(sp));
+        key = owned ? umax64(key, k_new) : key;
+        if (owned) {
+            const long long n = static_cast<long long>(i) * ys + j;
+            store_pair<TS>(RN2 + n, nr, nE);
+            store_pair<TS>(MN2 + n, nu, nv);
+        }
+
+        // Advance the march.
+        Pc = Pn;
+        rc = rn;
+        mc = mn;
+#pragma unroll
+        for (int q = 0; q < kPF; ++q) {
+            qr[q] = qr[q + 1];
+            qm[q] = qm[q + 1];
+        }
+        fx1_0 = fx2_0;
+        fx1_1 = fx2_1;
+        fx1_2 = fx2_2;
+        fx1_3 = fx2_3;
+    }
+    return key;
+}
+
+}  // namespace hydro

@@ -1,0 +1,279 @@
+// The 3D block march shared by the single-domain step kernel and the batched kernel.
+//
+// A workgroup of kNW = TY + 2 waves covers a (Y x Z) tile of TY rows x 62 columns and
+// marches along X over rows i0..i1 (reading i0-1 .. i1+1); wave w holds Y row
+// j0 - 1 + w (waves 0 and TY+1 are halo rows: they load and convert their row but
+// update nothing) and lane l holds Z column k:
+//   * the Z face (k, k+1) is computed once by lane k and shuffled to lane k+1;
+//   * the Y face (j, j+1) is computed once by the wave of row j from the primitives the
+//     wave of row j+1 left in LDS, and left in LDS for the wave of row j+1 (two
+//     workgroup barriers per X step, single-buffered LDS);
+//   * the X face between rows i and i+1 is carried in registers along the march.
+// Every wave of the workgroup must call march3d with the same i0, i1 (the barriers).
+// The arithmetic of every cell is the reference's (hydro3d/kernel.cxx:145-407) in its
+// operation order, so results do not depend on how the domain is tiled.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hydro_device.hpp"
+
+namespace hydro {
+
+constexpr int kOwnedZ = kWave - 2;
+constexpr int kTY = 6;               // owned Y rows per workgroup
+constexpr int kNW = kTY + 2;         // waves per workgroup
+constexpr int kBlock3 = kNW * kWave;
+
+template <typename T>
+struct Vec2_3;
+template <>
+struct Vec2_3<double> {
+    using type = double2;
+};
+template <>
+struct Vec2_3<float> {
+    using type = float2;
+};
+
+template <typename T>
+struct Trip {
+    T x, y, z;
+};
+
+
+
+template <typename T>
+struct Consts3 {
+    T C_min, gamma, gmo, divgamma, K;
+};
+
+__device__ __forceinline__ unsigned xcd_remap3(unsigned b, unsigned n)
+{
+    constexpr unsigned X = 8;
+    const unsigned q = n / X, r = n % X;
+    const unsigned xcd = b % X, k = b / X;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// Primitive state in the X frame: (rho, u, v, w, p, a).
+template <typename T>
+struct P6 {
+    T r, u, v, w, p, a;
+};
+
+template <typename T>
+__device__ __forceinline__ void load3(const typename Vec2_3<T>::type* __restrict__ R2,
+                                      const Trip<T>* __restrict__ M3, long long n, typename Vec2_3<T>::type& re,
+                                      Trip<T>& m)
+{
+    re = R2[n];
+    m = M3[n];
+}
+
+template <typename TS, typename TF, typename M>
+__device__ __forceinline__ void to_prim(const typename Vec2_3<TS>::type& re, const Trip<TS>& m, TF gmo, TF gamma,
+                                        P6<TF>& q)
+{
+    q.r = static_cast<TF>(re.x);
+    cons_to_prim3<TF, M>(q.r, static_cast<TF>(m.x), static_cast<TF>(m.y), static_cast<TF>(m.z), static_cast<TF>(re.y),
+                         gmo, gamma, q.u, q.v, q.w, q.p, q.a);
+}
+
+template <typename T>
+__device__ __forceinline__ Prim3<T> frame_x(const P6<T>& q)
+{
+    return Prim3<T>{ q.r, q.u, q.v, q.w, q.p, q.a };
+}
+template <typename T>
+__device__ __forceinline__ Prim3<T> frame_y(const P6<T>& q)
+{
+    return Prim3<T>{ q.r, q.v, q.u, q.w, q.p, q.a };  // hydro3d/kernel.cxx:254-256
+}
+template <typename T>
+__device__ __forceinline__ Prim3<T> frame_z(const P6<T>& q)
+{
+    return Prim3<T>{ q.r, q.w, q.v, q.u, q.p, q.a };  // hydro3d/kernel.cxx:296-298
+}
+
+// Row actually read for row r (see row_of in the 2D march): WRAP_X reads the periodic X
+// ghost planes from the opposite interior planes.
+template <bool WRAP_X>
+__device__ __forceinline__ long long row3_of(int r, int nb_x)
+{
+    if constexpr (WRAP_X) return r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r);
+    else return r;
+}
+
+template <typename TF>
+struct March3Lds {
+    TF prim[kNW][6][kWave];  // Y-frame primitives (rho, v, u, w, p, a) of every wave
+    TF flux[kNW][5][kWave];  // Y-face fluxes of every wave
+};
+
+// March rows i0..i1 of one (Y x Z) tile; see the file comment.  `col` is the in-plane
+// offset jl * zs + kl of the column this lane loads; stores go to the same column.
+template <typename TS, typename TF, typename M, bool WRAP_X>
+__device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __restrict__ R2,
+                                            const Trip<TS>* __restrict__ M3,
+                                            typename Vec2_3<TS>::type* __restrict__ RN2,
+                                            Trip<TS>* __restrict__ MN3, long long zy, long long col, int nb_x, int i0,
+                                            int i1, int w, int lane, bool halo_wave, bool owned, unsigned int ymask,
+                                            unsigned int zmask, TF gamma, TF gmo, TF divgamma, TF K, TS DtDx, TS DtDy,
+                                            TS DtDz, March3Lds<TF>& lds, uint64_t key)
+{
+    using V2 = typename Vec2_3<TS>::type;
+    auto& s_prim = lds.prim;
+    auto& s_flux = lds.flux;
+    V2 rc, rn;
+    Trip<TS> mc, mn;
+    P6<TF> Pc;
+    TF fx1_0 = TF(0), fx1_1 = TF(0), fx1_2 = TF(0), fx1_3 = TF(0), fx1_4 = TF(0);
+    load3<TS>(R2, M3, row3_of<WRAP_X>(i0, nb_x) * zy + col, rc, mc);
+    load3<TS>(R2, M3, row3_of<WRAP_X>(i0 + 1, nb_x) * zy + col, rn, mn);
+    to_prim<TS, TF, M>(rc, mc, gmo, gamma, Pc);
+    if (!halo_wave) {
+        V2 ra;
+        Trip<TS> ma;
+        P6<TF> Pm;
+        load3<TS>(R2, M3, row3_of<WRAP_X>(i0 - 1, nb_x) * zy + col, ra, ma);
+        to_prim<TS, TF, M>(ra, ma, gmo, gamma, Pm);
+        face_flux3<TF, M>(frame_x(Pm), frame_x(Pc), K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3, fx1_4);
+    }
+
+    for (int i = i0; i <= i1; ++i) {
+        V2 rnn;
+        Trip<TS> mnn;
+        const int ipf = min(i + 2, i1 + 1);
+        load3<TS>(R2, M3, row3_of<WRAP_X>(ipf, nb_x) * zy + col, rnn, mnn);
+
+        // (a) publish this row's Y-frame primitives.
+        const Prim3<TF> Sy = frame_y(Pc);
+        s_prim[w][0][lane] = Sy.r;
+        s_prim[w][1][lane] = xor_sign(Sy.un, ymask);
+        s_prim[w][2][lane] = Sy.ut1;
+        s_prim[w][3][lane] = Sy.ut2;
+        s_prim[w][4][lane] = Sy.p;
+        s_prim[w][5][lane] = Sy.a;
+        __syncthreads();
+
+        // (b) Y face (j, j+1), hydro3d/kernel.cxx:239-278, evaluated by the wave of row j
+        // from the primitives the wave of row j+1 published.
+        if (w < kNW - 1) {
+            Prim3<TF> L, R;
+            L.r = s_prim[w][0][lane];
+            L.un = s_prim[w][1][lane];
+            L.ut1 = s_prim[w][2][lane];
+            L.ut2 = s_prim[w][3][lane];
+            L.p = s_prim[w][4][lane];
+            L.a = s_prim[w][5][lane];
+            R.r = s_prim[w + 1][0][lane];
+            R.un = s_prim[w + 1][1][lane];
+            R.ut1 = s_prim[w + 1][2][lane];
+            R.ut2 = s_prim[w + 1][3][lane];
+            R.p = s_prim[w + 1][4][lane];
+            R.a = s_prim[w + 1][5][lane];
+            TF f0, f1, f2, f3, f4;
+            face_flux3<TF, M>(L, R, K, divgamma, f0, f1, f2, f3, f4);
+            s_flux[w][0][lane] = f0;
+            s_flux[w][1][lane] = f1;
+            s_flux[w][2][lane] = f2;
+            s_flux[w][3][lane] = f3;
+            s_flux[w][4][lane] = f4;
+        }
+        __syncthreads();
+
+        V2 rcur = rc;
+        Trip<TS> mcur = mc;
+        P6<TF> Pn;
+        to_prim<TS, TF, M>(rn, mn, gmo, gamma, Pn);
+        if (!halo_wave) {
+            // (c) Y face above this row, from the wave of row j-1.
+            const TF fy1_0 = s_flux[w - 1][0][lane];
+            const TF fy1_1 = s_flux[w - 1][1][lane];
+            const TF fy1_2 = s_flux[w - 1][2][lane];
+            const TF fy1_3 = s_flux[w - 1][3][lane];
+            const TF fy1_4 = s_flux[w - 1][4][lane];
+            const TF fy2_0 = s_flux[w][0][lane];
+            const TF fy2_1 = s_flux[w][1][lane];
+            const TF fy2_2 = s_flux[w][2][lane];
+            const TF fy2_3 = s_flux[w][3][lane];
+            const TF fy2_4 = s_flux[w][4][lane];
+
+            // Z face (k, k+1) by lane k, shuffled to lane k+1, hydro3d/kernel.cxx:281-320.
+            Prim3<TF> Sz = frame_z(Pc);
+            Sz.un = xor_sign(Sz.un, zmask);
+            Prim3<TF> Nz;
+            Nz.r = from_next_lane(Sz.r);
+            Nz.un = from_next_lane(Sz.un);
+            Nz.ut1 = from_next_lane(Sz.ut1);
+            Nz.ut2 = from_next_lane(Sz.ut2);
+            Nz.p = from_next_lane(Sz.p);
+            Nz.a = from_next_lane(Sz.a);
+            TF fz2_0, fz2_1, fz2_2, fz2_3, fz2_4;
+            face_flux3<TF, M>(Sz, Nz, K, divgamma, fz2_0, fz2_1, fz2_2, fz2_3, fz2_4);
+            const TF fz1_0 = from_prev_lane(fz2_0);
+            const TF fz1_1 = from_prev_lane(fz2_1);
+            const TF fz1_2 = from_prev_lane(fz2_2);
+            const TF fz1_3 = from_prev_lane(fz2_3);
+            const TF fz1_4 = from_prev_lane(fz2_4);
+
+            // X face (i, i+1).
+            TF fx2_0, fx2_1, fx2_2, fx2_3, fx2_4;
+            face_flux3<TF, M>(frame_x(Pc), frame_x(Pn), K, divgamma, fx2_0, fx2_1, fx2_2, fx2_3, fx2_4);
+
+            // Update, hydro3d/kernel.cxx:330-358: U' = ((U + x) + y) + z, in TS.
+            const TS x_r = DtDx * (static_cast<TS>(fx1_0) - static_cast<TS>(fx2_0));
+            const TS x_u = DtDx * (static_cast<TS>(fx1_1) - static_cast<TS>(fx2_1));
+            const TS x_v = DtDx * (static_cast<TS>(fx1_2) - static_cast<TS>(fx2_2));
+            const TS x_w = DtDx * (static_cast<TS>(fx1_3) - static_cast<TS>(fx2_3));
+            const TS x_E = DtDx * (static_cast<TS>(fx1_4) - static_cast<TS>(fx2_4));
+            const TS y_r = DtDy * (static_cast<TS>(fy1_0) - static_cast<TS>(fy2_0));
+            const TS y_u = DtDy * (static_cast<TS>(fy1_2) - static_cast<TS>(fy2_2));
+            const TS y_v = DtDy * (static_cast<TS>(fy1_1) - static_cast<TS>(fy2_1));
+            const TS y_w = DtDy * (static_cast<TS>(fy1_3) - static_cast<TS>(fy2_3));
+            const TS y_E = DtDy * (static_cast<TS>(fy1_4) - static_cast<TS>(fy2_4));
+            const TS z_r = DtDz * (static_cast<TS>(fz1_0) - static_cast<TS>(fz2_0));
+            const TS z_u = DtDz * (static_cast<TS>(fz1_3) - static_cast<TS>(fz2_3));
+            const TS z_v = DtDz * (static_cast<TS>(fz1_2) - static_cast<TS>(fz2_2));
+            const TS z_w = DtDz * (static_cast<TS>(fz1_1) - static_cast<TS>(fz2_1));
+            const TS z_E = DtDz * (static_cast<TS>(fz1_4) - static_cast<TS>(fz2_4));
+            const TS nr = rcur.x + x_r + y_r + z_r;
+            const TS nu = mcur.x + x_u + y_u + z_u;
+            const TS nv = mcur.y + x_v + y_v + z_v;
+            const TS nw = mcur.z + x_w + y_w + z_w;
+            const TS nE = rcur.y + x_E + y_E + z_E;
+            const TF sp = signal_speed3<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
+                                               static_cast<TF>(nw), static_cast<TF>(nE), gmo, gamma);
+            const uint64_t k_new = speed_key(static_cast<double>(sp));
+            key = owned ? umax64(key, k_new) : key;
+            if (owned) {
+                const long long n = static_cast<long long>(i) * zy + col;
+                V2 o;
+                o.x = nr;
+                o.y = nE;
+                RN2[n] = o;
+                Trip<TS> m;
+                m.x = nu;
+                m.y = nv;
+                m.z = nw;
+                MN3[n] = m;
+            }
+            fx1_0 = fx2_0;
+            fx1_1 = fx2_1;
+            fx1_2 = fx2_2;
+            fx1_3 = fx2_3;
+            fx1_4 = fx2_4;
+        }
+        Pc = Pn;
+        rc = rn;
+        mc = mn;
+        rn = rnn;
+        mn = mnn;
+    }
+
+    return key;
+}
+
+}  // namespace hydro

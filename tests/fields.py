@@ -1,5 +1,12 @@
 """Primitive fields of a conserved state and the per-field error measures used to compare
-forms of the solver (relative L-inf and relative L2 over interior cells)."""
+forms of the solver (relative L-inf and relative L2 over interior cells).
+
+Scales: rho and p are compared relative to their own maximum magnitude.  Velocity
+components are compared relative to the reference's maximum signal speed
+max(sqrt(gamma*p/rho) + |u|), the velocity scale of the solve (it sets Dt): a component
+that vanishes by symmetry (v in a one-cell-wide channel, u across a one-row periodic
+domain) holds only round-off, and an error relative to its own maximum would measure
+nothing."""
 import numpy as np
 
 GAMMA = 1.4
@@ -18,13 +25,21 @@ def primitives(rhoE, mom):
     return dict(zip(names, [rho, p] + vel))
 
 
+def signal_speed_scale(prims):
+    speed = np.sqrt(sum(prims[k] ** 2 for k in ("u", "v", "w") if k in prims))
+    return float(np.max(np.sqrt(GAMMA * np.abs(prims["p"]) / prims["rho"]) + speed))
+
+
 def field_errors(got, want):
-    """{field: (rel L-inf, rel L2)}: max|a-b| / max|b| and ||a-b||_2 / ||b||_2."""
+    """{field: (rel L-inf, rel L2)} against `want`: max|a-b| / scale and
+    ||a-b||_2 / (scale * sqrt(n)), with the scales of the module docstring."""
     a, b = primitives(*got), primitives(*want)
+    vscale = max(signal_speed_scale(b), 1e-300)
     out = {}
     for k in b:
         d = a[k] - b[k]
-        linf = float(np.max(np.abs(d)) / max(np.max(np.abs(b[k])), 1e-300))
-        l2 = float(np.sqrt(np.sum(d * d)) / max(np.sqrt(np.sum(b[k] * b[k])), 1e-300))
+        scale = vscale if k in ("u", "v", "w") else max(float(np.max(np.abs(b[k]))), 1e-300)
+        linf = float(np.max(np.abs(d)) / scale)
+        l2 = float(np.sqrt(np.mean(d * d)) / scale)
         out[k] = (linf, l2)
     return out

@@ -1,0 +1,209 @@
+"""The batched form: blasts of uneven sizes advanced together by one persistent kernel
+per step.  Every blast must be bit-identical to the oracle's run of the same blast (the
+reference step loop, hydro2d/main.cxx:686-815, with X periodic and Y/Z reflective),
+whatever the tiling, with or without graph replay, and however the batch's membership
+changes; blasts stop on their own max_it / max_t."""
+import numpy as np
+import pytest
+import torch
+
+from fields import field_errors
+
+pytestmark = pytest.mark.gpu
+
+SIZES_2D = [(1, 3), (5, 1), (7, 5), (64, 64), (130, 70), (63, 125), (200, 333), (31, 62), (17, 63), (300, 129)]
+SIZES_3D = [(9, 7, 5), (20, 17, 9), (33, 40, 70), (64, 13, 65), (3, 64, 2)]
+
+
+def _bufs(nb):
+    shape = tuple(n + 2 for n in nb)
+    a = [torch.zeros(shape + (2,), dtype=torch.float64, device="cuda"),
+         torch.zeros(shape + (len(nb),), dtype=torch.float64, device="cuda")]
+    b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+    return a, b
+
+
+def _params(i):
+    rng = np.random.default_rng(1000 + i)
+    return float(rng.uniform(0.05, 0.2)), float(0.1 * 10 ** rng.uniform(1, 3))
+
+
+def _oracle(oracle, nb, C, rf, p_in, steps, max_t=1e30):
+    if len(nb) == 2:
+        rhoE, mom, dt0 = oracle.ic2d(*nb, C_cfl=C, ic_mode=1, radius_frac=rf, p_in=p_in, p_out=0.1)
+        res = oracle.run2d(rhoE, mom, dt0, max_t, steps, C_cfl=C, nthreads=8)
+    else:
+        rhoE, mom, dt0 = oracle.ic3d(*nb, C_cfl=C, ic_mode=1, radius_frac=rf, p_in=p_in, p_out=0.1)
+        res = oracle.run3d(rhoE, mom, dt0, max_t, steps, C_cfl=C, nthreads=8)
+    return rhoE, mom, res
+
+
+def _interior(x, dim):
+    return x[(slice(1, -1),) * dim]
+
+
+def _check_slot(batch, slot, nb, ref, tag):
+    rhoE, mom, res = ref
+    dim = len(nb)
+    st = batch.status(slot)
+    assert st["rc"] == 0, (tag, st)
+    assert st["it"] == res["it"], (tag, st, res["it"])
+    assert st["t"] == res["t"], (tag, st["t"], res["t"])
+    assert st["dt"] == res["dt"], (tag, st["dt"], res["dt"])
+    cur = batch.current(slot)
+    g_r = _interior(cur[0].cpu().numpy(), dim)
+    g_m = _interior(cur[1].cpu().numpy(), dim)
+    np.testing.assert_array_equal(g_r.view(np.uint64), _interior(rhoE, dim).view(np.uint64), err_msg=tag)
+    np.testing.assert_array_equal(g_m.view(np.uint64), _interior(mom, dim).view(np.uint64), err_msg=tag)
+
+
+def _make(gpu, dim, sizes, steps, precision=None, capacity=None, max_it=None):
+    C = 0.5 if dim == 2 else 0.4
+    batch = gpu.Batch(dim, precision if precision is not None else gpu.FP64, capacity or len(sizes))
+    slots = []
+    for i, nb in enumerate(sizes):
+        rf, p_in = _params(i)
+        a, b = _bufs(nb)
+        torch.cuda.synchronize()
+        mi = max_it[i] if max_it else steps
+        slots.append((batch.add_blast(a, b, C, rf, p_in, max_it=mi), nb, C, rf, p_in, mi))
+    return batch, slots
+
+
+def test_batch2d_matches_oracle_per_blast(gpu, oracle):
+    steps = 30
+    max_it = [steps if i % 3 else 7 + i for i in range(len(SIZES_2D))]
+    batch, slots = _make(gpu, 2, SIZES_2D, steps, max_it=max_it)
+    batch.advance(steps)
+    batch.sync()
+    flags = batch.done_flags(len(slots))
+    assert flags.tolist() == [1] * len(slots)
+    for slot, nb, C, rf, p_in, mi in slots:
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, mi), f"2D {nb} max_it {mi}")
+    batch.close()
+
+
+def test_batch2d_tiling_changes_mid_run(gpu, oracle):
+    batch, slots = _make(gpu, 2, SIZES_2D, 24)
+    for bpc, rows in [(1, 0), (4, 32), (2, 128), (3, 0)]:
+        batch.set_tiling(bpc, rows)
+        assert batch.tiling() == (bpc, rows)
+        batch.advance(6)
+    for slot, nb, C, rf, p_in, mi in slots:
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 24), f"2D tiling {nb}")
+    batch.close()
+
+
+def test_batch2d_graph_replay(gpu, oracle):
+    batch, slots = _make(gpu, 2, SIZES_2D[:6], 40)
+    batch.set_graph(4)
+    batch.advance(10)  # two replays + two direct steps
+    x = torch.randn(1 << 20, device="cuda")  # eager work between replays
+    torch.cuda.synchronize()
+    assert float((x * 0).sum()) == 0.0
+    batch.advance(9)
+    batch.sync()
+    batch.advance(8)
+    for slot, nb, C, rf, p_in, mi in slots:
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 27), f"2D graph {nb}")
+    batch.close()
+
+
+def test_batch2d_autotune_steps_are_solve_steps(gpu, oracle):
+    batch, slots = _make(gpu, 2, SIZES_2D, 40)
+    ms = batch.autotune(steps_per_trial=1, use_cache=False)
+    assert ms > 0
+    bpc, rows = batch.tiling()
+    ntrials = 4 * 4  # blocks per CU 1..4 x (balanced, 32, 64, 128)
+    batch.advance(30 - ntrials)
+    for slot, nb, C, rf, p_in, mi in slots:
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 30), f"2D tuned ({bpc},{rows}) {nb}")
+    # A second batch of the same signature reuses the tuned tiling without trials.
+    again, _ = _make(gpu, 2, SIZES_2D, 40)
+    assert again.autotune(steps_per_trial=1, use_cache=True) == -1.0
+    assert again.tiling() == (bpc, rows)
+    again.close()
+    batch.close()
+
+
+def test_batch_membership_changes(gpu, oracle):
+    C = 0.5
+    batch = gpu.Batch(2, gpu.FP64, 3)
+    blasts = {}
+    for i, nb in enumerate([(40, 30), (90, 64), (21, 200)]):
+        rf, p_in = _params(i)
+        a, b = _bufs(nb)
+        torch.cuda.synchronize()
+        blasts[batch.add_blast(a, b, C, rf, p_in, max_it=100)] = (nb, rf, p_in, 0)
+    batch.advance(5)
+    batch.remove(1)
+    del blasts[1]
+    assert batch.status(1)["rc"] == -1  # HYDRO_ERR_ARG: slot not in use
+    nb = (77, 33)
+    rf, p_in = _params(7)
+    a, b = _bufs(nb)
+    torch.cuda.synchronize()
+    s = batch.add_blast(a, b, C, rf, p_in, max_it=100)
+    assert s == 1
+    blasts[s] = (nb, rf, p_in, 5)
+    # Full: a fourth blast is refused.
+    a4, b4 = _bufs((8, 8))
+    torch.cuda.synchronize()
+    with pytest.raises(gpu.HydroError):
+        batch.add_blast(a4, b4, C, 0.1, 10.0)
+    batch.advance(10)
+    for slot, (nb, rf, p_in, start) in blasts.items():
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 15 - start), f"membership {nb}")
+    batch.close()
+
+
+def test_batch2d_stops_on_time(gpu, oracle):
+    C = 0.5
+    batch = gpu.Batch(2, gpu.FP64, 2)
+    rf, p_in = _params(3)
+    nb = (50, 40)
+    a, b = _bufs(nb)
+    torch.cuda.synchronize()
+    slot = batch.add_blast(a, b, C, rf, p_in, max_t=0.02, max_it=10000)
+    batch.advance(400)
+    ref = _oracle(oracle, nb, C, rf, p_in, 10000, max_t=0.02)
+    assert 0 < ref[2]["it"] < 400
+    _check_slot(batch, slot, nb, ref, "stop on t")
+    batch.close()
+
+
+def test_batch3d_matches_oracle_per_blast(gpu, oracle):
+    steps = 15
+    batch, slots = _make(gpu, 3, SIZES_3D, steps)
+    batch.advance(5)
+    batch.set_tiling(1, 4)
+    batch.advance(5)
+    batch.set_tiling(3, 0)
+    batch.advance(5)
+    for slot, nb, C, rf, p_in, mi in slots:
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, steps), f"3D {nb}")
+    batch.close()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_batch_fp32_flux_vs_fp64(gpu, dim):
+    sizes = SIZES_2D if dim == 2 else SIZES_3D
+    steps = 60 if dim == 2 else 30
+    ref, rs = _make(gpu, dim, sizes, steps, precision=gpu.FP64)
+    got, gs = _make(gpu, dim, sizes, steps, precision=gpu.FP32_FLUX)
+    ref.advance(steps)
+    got.advance(steps)
+    worst = {}
+    for (sr, nb, *_), (sg, *_) in zip(rs, gs):
+        a = tuple(x.cpu().numpy() for x in ref.current(sr))
+        b = tuple(x.cpu().numpy() for x in got.current(sg))
+        errs = field_errors(b, a)
+        print(f"  {nb}: " + ", ".join(f"{k} {v[0]:.1e}" for k, v in errs.items())
+              + f" | max|v_ref| " + ", ".join(f"{np.max(np.abs(a[1][..., c])):.1e}" for c in range(dim)), flush=True)
+        for k, (linf, _) in errs.items():
+            worst[k] = max(worst.get(k, 0.0), linf)
+    print(f"[batch fp32-flux {dim}D] worst rel Linf per field: {worst}", flush=True)
+    for k, v in worst.items():
+        assert v <= 1e-5, (k, v)
+    ref.close()
+    got.close()
