@@ -220,6 +220,81 @@ def bench_batched(args, name: str) -> dict:
     }
 
 
+def bench_served(args) -> dict:
+    """Poisson stream of mixed solves (70% 2D sides 128..4096, 30% 3D sides 64..256,
+    100..2000 steps, FP32-flux / FP64 50/50, seeded blasts) through the continuous-batching
+    server.  The arrival rate is set for 80% load of a server that solves one request at a
+    time on the plain device-resident loop: rate = 0.8 / mean plain solve time, from the
+    measured solve time of every request.  Reported: p50/p99 latency and throughput of the
+    served form at that rate, of the sequential server at the same arrivals (FIFO queueing
+    of the measured solve times), and the saturation throughput of both (all requests at
+    t = 0)."""
+    import torch
+
+    hydro = importlib.import_module("scalable-fpga-hydrodynamics_amd")
+    wl = importlib.import_module("scalable-fpga-hydrodynamics_amd.workloads")
+    sv = importlib.import_module("scalable-fpga-hydrodynamics_amd.serve")
+    n = args.requests
+    reqs = wl.served_requests(n, rate_per_s=1.0)
+    work = sum(sv.cell_updates(r) for r in reqs)
+    log(f"[bench] served: {n} requests, {work / 1e9:.1f} G cell-updates, "
+        f"{sum(1 for r in reqs if len(r['nb']) == 2)} 2D / {sum(1 for r in reqs if len(r['nb']) == 3)} 3D, "
+        f"{sum(1 for r in reqs if r['fp32flux'])} FP32-flux")
+    prog = lambda d, t: d % 20 == 0 and log(f"[bench]   {d}/{t}")  # noqa: E731
+    log("[bench] plain solves, one at a time ...")
+    dur = sv.plain_solve_times(hydro, torch, reqs, progress=prog)
+    torch.cuda.empty_cache()
+    mean_d = sum(dur) / n
+    rate = 0.8 / mean_d
+    for r in reqs:
+        r["arrival"] = r["arrival"] / rate
+    seq_lat = sv.sequential_latencies(reqs, dur)
+    seq_finish = max(r["arrival"] + l for r, l in zip(reqs, seq_lat))
+    arena = int(args.serve_hbm_gb * 2**30)
+    log(f"[bench] served at {rate:.2f} requests/s (80% of the sequential server) ...")
+    srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
+    live = sv.run_trace(hydro, srv, reqs, realtime=True, progress=prog)
+    srv.close()
+    log("[bench] served at saturation (all requests at t = 0) ...")
+    srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
+    sat = sv.run_trace(hydro, srv, reqs, realtime=False, progress=prog)
+    srv.close()
+    seq_gcups = work / sum(dur) / 1e9
+    line = {
+        "metric": "cell-updates/s (GCUPS) + p50/p99 solve latency when served",
+        "value": round(sat["gcups"], 4),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": None,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state; f32 flux for half the requests",
+        "data": f"synthetic Poisson stream, splitmix64 seed 0x5EED, {n} requests",
+        "config": {"workload": "served_mixed", "requests": n, "mix": "70% 2D 128..4096, 30% 3D 64..256",
+                   "steps": "100..2000", "precision": "50% fp32-flux / 50% fp64", "hbm_arena_gb": args.serve_hbm_gb,
+                   "steps_per_tick": args.steps_per_tick, "arrival_rate_per_s": round(rate, 3),
+                   "load": "0.8 of the one-at-a-time plain server"},
+        "served": {"p50_s": round(live["p50"], 4), "p99_s": round(live["p99"], 4), "gcups": round(live["gcups"], 3),
+                   "makespan_s": round(live["makespan"], 3)},
+        "sequential_plain": {"p50_s": round(sv.percentile(seq_lat, 50), 4),
+                             "p99_s": round(sv.percentile(seq_lat, 99), 4),
+                             "gcups": round(work / (seq_finish - reqs[0]["arrival"]) / 1e9, 3),
+                             "mean_solve_s": round(mean_d, 4)},
+        "saturation": {"served_gcups": round(sat["gcups"], 3), "sequential_plain_gcups": round(seq_gcups, 3),
+                       "speedup": round(sat["gcups"] / seq_gcups, 3), "served_p50_s": round(sat["p50"], 4),
+                       "served_p99_s": round(sat["p99"], 4)},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    log(f"[bench] served p50 {live['p50']:.3f} s p99 {live['p99']:.3f} s | sequential p50 "
+        f"{line['sequential_plain']['p50_s']} s p99 {line['sequential_plain']['p99_s']} s | saturation "
+        f"{sat['gcups']:.1f} vs {seq_gcups:.1f} GCUPS")
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +306,9 @@ def main() -> int:
     ap.add_argument("--no-copy-reference", action="store_true", help="skip the achievable-bandwidth copy")
     ap.add_argument("--plain-steps", type=int, default=20, help="steps per blast of the one-solve-at-a-time baselines")
     ap.add_argument("--plain-sync-blasts", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=120, help="served workload: number of requests")
+    ap.add_argument("--serve-hbm-gb", type=float, default=48.0, help="served workload: state arena (GiB)")
+    ap.add_argument("--steps-per-tick", type=int, default=50)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -241,6 +319,14 @@ def main() -> int:
         return 2
 
     name = ALIASES.get(args.workload, args.workload)
+    if name == "served":
+        import torch
+
+        if not torch.cuda.is_available():
+            log("no GPU visible")
+            return 1
+        print(json.dumps(bench_served(args)), flush=True)
+        return 0
     if name in BATCHED:
         import torch
 

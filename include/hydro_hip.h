@@ -239,6 +239,49 @@ int hydro_batch_set_graph(hydro_batch* batch, size_t steps_per_graph);
 int hydro_batch_autotune(hydro_batch* batch, size_t steps_per_trial, int use_cache, double* best_ms_per_step);
 
 /* ---------------------------------------------------------------------------------
+ * Served form: a request loop with continuous batching over the batched engines.
+ *
+ * No counterpart in the reference (one solve per process run).  Each request is one
+ * blast solve (as hydro_batch_add_blast).  Requests queue on submission and are admitted
+ * in arrival order as soon as their device footprint -- two FP64 buffer pairs with
+ * ghosts, 256-byte aligned -- fits in the server's HBM arena (one allocation made at
+ * create) and their engine has a free slot.  One engine per (dimension, precision), each
+ * on its own stream; a request joins between ticks and leaves when its own loop ends.
+ * hydro_server_tick: admit, advance every engine by steps_per_tick, wait, retire.  Times
+ * are seconds on the server's steady clock (hydro_server_now).
+ * ------------------------------------------------------------------------------- */
+
+typedef struct hydro_server hydro_server;
+
+/* hbm_bytes: size of the state arena; capacity: running requests per engine. */
+hydro_server* hydro_server_create(size_t hbm_bytes, size_t capacity, size_t steps_per_tick);
+void hydro_server_destroy(hydro_server* server);
+double hydro_server_now(hydro_server* server);
+
+/* Queue a request; precision HYDRO_FP64 or HYDRO_FP32_FLUX; keep_result != 0 keeps the
+ * final state in the arena until hydro_server_release (else it is freed on completion);
+ * arrival = its arrival time on the server clock (< 0: now).  Returns the request id (>= 0)
+ * or a negative error (HYDRO_ERR_NOMEM if it can never fit). */
+int hydro_server_submit(hydro_server* server, int dim, const size_t* nb, int precision, double C, double radius_frac,
+                        double rho_in, double rho_out, double p_in, double p_out, double max_t, size_t max_it,
+                        int keep_result, double arrival);
+
+/* One tick; *n_done = requests completed in it. */
+int hydro_server_tick(hydro_server* server, size_t* n_done);
+int hydro_server_pending(hydro_server* server, size_t* queued, size_t* running, size_t* bytes_in_use);
+
+/* state: 0 queued, 1 running, 2 done (result kept), 3 done (memory released), 4 failed
+ * (non-finite Dt).  Returns the request's error code (0 unless failed). */
+int hydro_server_request(hydro_server* server, int id, int* state, double* t_arrival, double* t_admit,
+                         double* t_done, size_t* it, double* t_sim);
+
+/* Copy a kept result (current buffer pair, reference layout, ghost cells undefined) to
+ * host or device memory; then release it. */
+int hydro_server_result(hydro_server* server, int id, void* dst_rhoE, void* dst_mom);
+int hydro_server_release(hydro_server* server, int id);
+int hydro_server_stats(hydro_server* server, size_t* ticks, double* cell_updates);
+
+/* ---------------------------------------------------------------------------------
  * Diagnostics.
  * ------------------------------------------------------------------------------- */
 

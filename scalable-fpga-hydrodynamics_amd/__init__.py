@@ -103,6 +103,19 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_batch_get_tiling.argtypes = [vp, vp, vp]
     L.hydro_batch_set_graph.argtypes = [vp, sz]
     L.hydro_batch_autotune.argtypes = [vp, sz, i32, vp]
+    L.hydro_server_create.argtypes = [sz, sz, sz]
+    L.hydro_server_create.restype = vp
+    L.hydro_server_destroy.argtypes = [vp]
+    L.hydro_server_destroy.restype = None
+    L.hydro_server_now.argtypes = [vp]
+    L.hydro_server_now.restype = d
+    L.hydro_server_submit.argtypes = [vp, i32, vp, i32, d, d, d, d, d, d, d, sz, i32, d]
+    L.hydro_server_tick.argtypes = [vp, vp]
+    L.hydro_server_pending.argtypes = [vp, vp, vp, vp]
+    L.hydro_server_request.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp]
+    L.hydro_server_result.argtypes = [vp, i32, vp, vp]
+    L.hydro_server_release.argtypes = [vp, i32]
+    L.hydro_server_stats.argtypes = [vp, vp, vp]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
 
@@ -396,6 +409,75 @@ class Batch:
             lib().hydro_batch_destroy(self._h)
             self._h = None
             self._bufs = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (include/hydro_hip.h, hydro_server_*)."""
+
+    QUEUED, RUNNING, DONE, RELEASED, FAILED = range(5)
+
+    def __init__(self, hbm_bytes: int, capacity: int = 256, steps_per_tick: int = 50):
+        h = lib().hydro_server_create(int(hbm_bytes), int(capacity), int(steps_per_tick))
+        if not h:
+            raise HydroError(f"hydro_server_create failed: {last_error()}")
+        self._h = C.c_void_p(h)
+
+    def now(self) -> float:
+        return lib().hydro_server_now(self._h)
+
+    def submit(self, nb, precision=FP64, C=None, radius_frac=0.1, p_in=10.0, p_out=0.1, rho_in=1.0, rho_out=1.2,
+               max_t=1e30, max_it=100, keep=False, arrival=-1.0) -> int:
+        dim = len(nb)
+        C = (0.5 if dim == 2 else 0.4) if C is None else C
+        rid = lib().hydro_server_submit(self._h, dim, _sizes(nb), int(precision), float(C), float(radius_frac),
+                                        float(rho_in), float(rho_out), float(p_in), float(p_out), float(max_t),
+                                        int(max_it), 1 if keep else 0, float(arrival))
+        if rid < 0:
+            raise HydroError(f"hydro_server_submit failed ({rid}): {last_error()}")
+        return rid
+
+    def tick(self) -> int:
+        n = C.c_size_t()
+        _check(lib().hydro_server_tick(self._h, C.byref(n)), "hydro_server_tick")
+        return n.value
+
+    def pending(self):
+        q, r, b = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _check(lib().hydro_server_pending(self._h, C.byref(q), C.byref(r), C.byref(b)), "hydro_server_pending")
+        return q.value, r.value, b.value
+
+    def request(self, rid: int) -> dict:
+        st, ta, tad, td, it, ts = C.c_int(), C.c_double(), C.c_double(), C.c_double(), C.c_size_t(), C.c_double()
+        rc = lib().hydro_server_request(self._h, int(rid), C.byref(st), C.byref(ta), C.byref(tad), C.byref(td),
+                                        C.byref(it), C.byref(ts))
+        return {"state": st.value, "arrival": ta.value, "admit": tad.value, "done": td.value, "it": it.value,
+                "t": ts.value, "rc": rc}
+
+    def result(self, rid: int, rhoE, mom) -> None:
+        """Copy a kept result into (rhoE, mom) (device or host tensors / numpy arrays)."""
+        def ptr(x):
+            return C.c_void_p(x.data_ptr()) if hasattr(x, "data_ptr") else x.ctypes.data_as(C.c_void_p)
+
+        _check(lib().hydro_server_result(self._h, int(rid), ptr(rhoE), ptr(mom)), "hydro_server_result")
+
+    def release(self, rid: int) -> None:
+        _check(lib().hydro_server_release(self._h, int(rid)), "hydro_server_release")
+
+    def stats(self):
+        t, cu = C.c_size_t(), C.c_double()
+        _check(lib().hydro_server_stats(self._h, C.byref(t), C.byref(cu)), "hydro_server_stats")
+        return t.value, cu.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().hydro_server_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

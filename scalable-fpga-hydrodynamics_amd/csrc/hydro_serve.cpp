@@ -1,0 +1,397 @@
+// Served form: a request loop over the batched engines with continuous batching.
+//
+// Requests (one blast solve each: dimension, size, precision, blast parameters, stop
+// conditions) are queued on arrival and admitted in arrival order as soon as their
+// device footprint (two FP64 buffer pairs, ghosts included) fits in the server's HBM
+// arena and their engine has a free slot.  One batch engine per (dimension, precision)
+// advances all of its running requests together, each on its own stream, so requests of
+// every kind overlap on the device; a request joins its engine between ticks and leaves
+// it when its own loop ends (max_it or max_t, hydro2d/main.cxx:689), freeing memory for
+// the queue.  State stays on the device from the initial condition (written there) to
+// the end; a result is copied out only if the client asks for it.
+//
+// A tick: admit what fits, enqueue steps_per_tick steps on every engine with running
+// requests, wait, retire finished requests.  Latency = completion observed at the end
+// of a tick - arrival, on the server's steady clock.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <deque>
+#include <map>
+#include <new>
+#include <string.h>
+#include <vector>
+
+#include "../../include/hydro_hip.h"
+#include "hydro_runtime.hpp"
+
+using namespace hydro;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+// First-fit allocator over [0, size) with coalescing free list (offset -> length).
+class Arena {
+public:
+    void reset(size_t size)
+    {
+        free_.clear();
+        if (size) free_[0] = size;
+        size_ = size;
+        used_ = 0;
+    }
+    bool alloc(size_t bytes, size_t& off)
+    {
+        bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            off = it->first;
+            const size_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            used_ += bytes;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes)
+    {
+        bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+    size_t size() const { return size_; }
+
+private:
+    std::map<size_t, size_t> free_;
+    size_t size_ = 0, used_ = 0;
+};
+
+enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kReleased = 3, kFailed = 4 };
+
+struct Req {
+    int dim = 2;
+    size_t nb[3] = { 1, 1, 1 };
+    int precision = HYDRO_FP64;
+    double C = 0.5, rf = 0.1, rho_in = 1.0, rho_out = 1.2, p_in = 10.0, p_out = 0.1, max_t = 1e30;
+    size_t max_it = 0;
+    bool keep = false;
+    int state = kQueued;
+    double t_arrival = 0, t_admit = -1, t_done = -1;
+    size_t off = 0, bytes = 0, buf_bytes[2] = { 0, 0 };  // rhoE / momentum bytes per buffer
+    int engine = -1, slot = -1, current = 0, rc = 0;
+    size_t it = 0;
+    double t_sim = 0;
+    double interior = 0;  // interior cells
+};
+
+size_t cells_of(const Req& r)
+{
+    size_t c = 1;
+    for (int d = 0; d < r.dim; ++d) c *= r.nb[d] + 2;
+    return c;
+}
+
+int engine_of(const Req& r) { return (r.dim == 3 ? 2 : 0) + (r.precision == HYDRO_FP32_FLUX ? 1 : 0); }
+
+}  // namespace
+
+struct hydro_server {
+    int device = 0;
+    char* base = nullptr;
+    Arena arena;
+    size_t capacity = 0;  // running requests per engine
+    size_t steps_per_tick = 50;
+    hydro_batch* eng[4] = { nullptr, nullptr, nullptr, nullptr };
+    std::vector<int> slot_req[4];  // slot -> request id (-1 free)
+    size_t running[4] = { 0, 0, 0, 0 };
+    std::vector<Req> reqs;
+    std::deque<int> queue;
+    std::chrono::steady_clock::time_point t0;
+    size_t ticks = 0;
+    double cell_updates = 0;  // of retired requests
+
+    double now() const
+    {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+bool check(hydro_server* s, const char* fn)
+{
+    clear_error();
+    if (!s) {
+        set_error("%s: null server", fn);
+        return false;
+    }
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != s->device) {
+        set_error("%s: the server belongs to device %d, the current device is %d", fn, s->device, dev);
+        return false;
+    }
+    return true;
+}
+
+int ensure_engine(hydro_server* s, int e)
+{
+    if (s->eng[e]) return 0;
+    const int dim = e >= 2 ? 3 : 2;
+    const int prec = (e & 1) ? HYDRO_FP32_FLUX : HYDRO_FP64;
+    s->eng[e] = hydro_batch_create(dim, prec, s->capacity);
+    if (!s->eng[e]) return HYDRO_ERR_HIP;
+    s->slot_req[e].assign(s->capacity, -1);
+    // Continuous batching replays a few steps at a time from a graph; tiles of 32 rows
+    // balance well across the changing mix.
+    int rc = hydro_batch_set_graph(s->eng[e], 10);
+    if (rc == 0) rc = hydro_batch_set_tiling(s->eng[e], 2, dim == 2 ? 32 : 0);
+    return rc;
+}
+
+// Admit queued requests in arrival order while the head fits (memory and a slot).
+int admit(hydro_server* s)
+{
+    while (!s->queue.empty()) {
+        Req& r = s->reqs[s->queue.front()];
+        const int e = engine_of(r);
+        int rc = ensure_engine(s, e);
+        if (rc) return rc;
+        if (s->running[e] >= s->capacity) break;
+        size_t off = 0;
+        if (!s->arena.alloc(r.bytes, off)) break;
+        char* p = s->base + off;
+        void* ra = p;
+        void* ma = p + r.buf_bytes[0];
+        void* rb = p + r.buf_bytes[0] + r.buf_bytes[1];
+        void* mb = p + 2 * r.buf_bytes[0] + r.buf_bytes[1];
+        const int slot = hydro_batch_add_blast(s->eng[e], r.nb, r.C, r.rf, r.rho_in, r.rho_out, r.p_in, r.p_out,
+                                               r.max_t, r.max_it, ra, ma, rb, mb);
+        if (slot < 0) {
+            s->arena.release(off, r.bytes);
+            return slot;
+        }
+        r.off = off;
+        r.engine = e;
+        r.slot = slot;
+        r.state = kRunning;
+        r.t_admit = s->now();
+        s->slot_req[e][slot] = s->queue.front();
+        ++s->running[e];
+        s->queue.pop_front();
+    }
+    return 0;
+}
+
+void release_memory(hydro_server* s, Req& r)
+{
+    s->arena.release(r.off, r.bytes);
+    r.state = r.state == kFailed ? kFailed : kReleased;
+}
+
+int retire(hydro_server* s, size_t* n_done)
+{
+    std::vector<unsigned char> flags(s->capacity);
+    for (int e = 0; e < 4; ++e) {
+        if (!s->eng[e] || s->running[e] == 0) continue;
+        int rc = hydro_batch_done_flags(s->eng[e], flags.data(), s->capacity);
+        if (rc) return rc;
+        const double t = s->now();
+        for (size_t slot = 0; slot < s->capacity; ++slot) {
+            const int id = s->slot_req[e][slot];
+            if (id < 0 || !flags[slot]) continue;
+            Req& r = s->reqs[id];
+            int done = 0;
+            r.rc = hydro_batch_status(s->eng[e], static_cast<int>(slot), &r.it, &r.t_sim, nullptr, &done, &r.current);
+            r.t_done = t;
+            r.state = r.rc == 0 ? kDone : kFailed;
+            s->cell_updates += static_cast<double>(r.it) * r.interior;
+            rc = hydro_batch_remove(s->eng[e], static_cast<int>(slot));
+            if (rc) return rc;
+            s->slot_req[e][slot] = -1;
+            --s->running[e];
+            if (!r.keep || r.state == kFailed) release_memory(s, r);
+            if (n_done) ++*n_done;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" hydro_server* hydro_server_create(size_t hbm_bytes, size_t capacity, size_t steps_per_tick)
+{
+    clear_error();
+    if (hbm_bytes < kAlign || capacity < 1 || capacity > (1u << 16) || steps_per_tick < 1) {
+        set_error("hydro_server_create: hbm_bytes >= 256, capacity in [1, 65536], steps_per_tick >= 1");
+        return nullptr;
+    }
+    hydro_server* s = new (std::nothrow) hydro_server();
+    if (!s) {
+        set_error("hydro_server_create: out of host memory");
+        return nullptr;
+    }
+    s->capacity = capacity;
+    s->steps_per_tick = steps_per_tick;
+    s->t0 = std::chrono::steady_clock::now();
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->base), hbm_bytes);
+    if (e != hipSuccess) {
+        set_error("hydro_server_create: %s (%zu bytes)", hipGetErrorString(e), hbm_bytes);
+        delete s;
+        return nullptr;
+    }
+    s->arena.reset(hbm_bytes);
+    return s;
+}
+
+extern "C" void hydro_server_destroy(hydro_server* s)
+{
+    if (!s) return;
+    for (int e = 0; e < 4; ++e)
+        if (s->eng[e]) hydro_batch_destroy(s->eng[e]);
+    if (s->base) (void)hipFree(s->base);
+    delete s;
+}
+
+extern "C" double hydro_server_now(hydro_server* s) { return s ? s->now() : 0.0; }
+
+extern "C" int hydro_server_submit(hydro_server* s, int dim, const size_t* nb, int precision, double C,
+                                   double radius_frac, double rho_in, double rho_out, double p_in, double p_out,
+                                   double max_t, size_t max_it, int keep_result, double arrival)
+{
+    if (!check(s, "hydro_server_submit")) return HYDRO_ERR_ARG;
+    if ((dim != 2 && dim != 3) || !nb || (precision != HYDRO_FP64 && precision != HYDRO_FP32_FLUX)) {
+        set_error("hydro_server_submit: dim 2 or 3, nb non-null, precision HYDRO_FP64 or HYDRO_FP32_FLUX");
+        return HYDRO_ERR_ARG;
+    }
+    Req r;
+    r.dim = dim;
+    for (int d = 0; d < dim; ++d) {
+        if (nb[d] < 1 || nb[d] > (1u << 30)) {
+            set_error("hydro_server_submit: nb[%d] = %zu out of [1, 2^30]", d, nb[d]);
+            return HYDRO_ERR_ARG;
+        }
+        r.nb[d] = nb[d];
+    }
+    r.precision = precision;
+    r.C = C;
+    r.rf = radius_frac;
+    r.rho_in = rho_in;
+    r.rho_out = rho_out;
+    r.p_in = p_in;
+    r.p_out = p_out;
+    r.max_t = max_t;
+    r.max_it = max_it;
+    r.keep = keep_result != 0;
+    r.t_arrival = arrival >= 0 ? arrival : s->now();
+    const size_t cells = cells_of(r);
+    r.interior = 1;
+    for (int d = 0; d < dim; ++d) r.interior *= static_cast<double>(r.nb[d]);
+    // Each of the four arrays starts on a 256-byte boundary of the arena.
+    r.buf_bytes[0] = (cells * 2 * 8 + kAlign - 1) / kAlign * kAlign;
+    r.buf_bytes[1] = (cells * static_cast<size_t>(dim) * 8 + kAlign - 1) / kAlign * kAlign;
+    r.bytes = 2 * (r.buf_bytes[0] + r.buf_bytes[1]);
+    if (r.bytes > s->arena.size()) {
+        set_error("hydro_server_submit: request needs %zu bytes, more than the server's %zu", r.bytes,
+                  s->arena.size());
+        return HYDRO_ERR_NOMEM;
+    }
+    s->reqs.push_back(r);
+    const int id = static_cast<int>(s->reqs.size() - 1);
+    s->queue.push_back(id);
+    return id;
+}
+
+extern "C" int hydro_server_tick(hydro_server* s, size_t* n_done)
+{
+    if (!check(s, "hydro_server_tick")) return HYDRO_ERR_ARG;
+    if (n_done) *n_done = 0;
+    int rc = admit(s);
+    if (rc) return rc;
+    for (int e = 0; e < 4; ++e)
+        if (s->eng[e] && s->running[e] > 0) {
+            rc = hydro_batch_advance(s->eng[e], s->steps_per_tick);
+            if (rc) return rc;
+        }
+    ++s->ticks;
+    return retire(s, n_done);
+}
+
+extern "C" int hydro_server_pending(hydro_server* s, size_t* queued, size_t* running, size_t* bytes_in_use)
+{
+    if (!check(s, "hydro_server_pending")) return HYDRO_ERR_ARG;
+    if (queued) *queued = s->queue.size();
+    if (running) *running = s->running[0] + s->running[1] + s->running[2] + s->running[3];
+    if (bytes_in_use) *bytes_in_use = s->arena.used();
+    return 0;
+}
+
+extern "C" int hydro_server_request(hydro_server* s, int id, int* state, double* t_arrival, double* t_admit,
+                                    double* t_done, size_t* it, double* t_sim)
+{
+    if (!check(s, "hydro_server_request")) return HYDRO_ERR_ARG;
+    if (id < 0 || static_cast<size_t>(id) >= s->reqs.size()) {
+        set_error("hydro_server_request: unknown request %d", id);
+        return HYDRO_ERR_ARG;
+    }
+    const Req& r = s->reqs[id];
+    if (state) *state = r.state;
+    if (t_arrival) *t_arrival = r.t_arrival;
+    if (t_admit) *t_admit = r.t_admit;
+    if (t_done) *t_done = r.t_done;
+    if (it) *it = r.it;
+    if (t_sim) *t_sim = r.t_sim;
+    return r.rc;
+}
+
+extern "C" int hydro_server_result(hydro_server* s, int id, void* dst_rhoE, void* dst_mom)
+{
+    if (!check(s, "hydro_server_result")) return HYDRO_ERR_ARG;
+    if (id < 0 || static_cast<size_t>(id) >= s->reqs.size() || s->reqs[id].state != kDone) {
+        set_error("hydro_server_result: request %d has no kept result", id);
+        return HYDRO_ERR_ARG;
+    }
+    const Req& r = s->reqs[id];
+    const char* p = s->base + r.off;
+    const char* rhoE = p + (r.current ? r.buf_bytes[0] + r.buf_bytes[1] : 0);
+    const char* mom = rhoE + r.buf_bytes[0];
+    const size_t cells = cells_of(r);
+    if (dst_rhoE) HYDRO_HIP_OR(hipMemcpy(dst_rhoE, rhoE, cells * 2 * 8, hipMemcpyDefault), HYDRO_ERR_HIP);
+    if (dst_mom)
+        HYDRO_HIP_OR(hipMemcpy(dst_mom, mom, cells * static_cast<size_t>(r.dim) * 8, hipMemcpyDefault), HYDRO_ERR_HIP);
+    return 0;
+}
+
+extern "C" int hydro_server_release(hydro_server* s, int id)
+{
+    if (!check(s, "hydro_server_release")) return HYDRO_ERR_ARG;
+    if (id < 0 || static_cast<size_t>(id) >= s->reqs.size() || s->reqs[id].state != kDone) {
+        set_error("hydro_server_release: request %d has no kept result", id);
+        return HYDRO_ERR_ARG;
+    }
+    release_memory(s, s->reqs[id]);
+    return 0;
+}
+
+extern "C" int hydro_server_stats(hydro_server* s, size_t* ticks, double* cell_updates)
+{
+    if (!check(s, "hydro_server_stats")) return HYDRO_ERR_ARG;
+    if (ticks) *ticks = s->ticks;
+    if (cell_updates) *cell_updates = s->cell_updates;
+    return 0;
+}

@@ -1,0 +1,106 @@
+"""Drive the served form with a request stream and measure it.
+
+run_trace replays requests (scalable-fpga-hydrodynamics_amd.workloads.served_requests)
+against a Server: in real time (a request is submitted once the server clock passes its
+arrival time, and its latency counts from that arrival) or all at once (saturation).
+sequential_latencies gives what a server that solves one request at a time, in arrival
+order, would show for the same arrivals and measured per-request solve times.
+"""
+from __future__ import annotations
+
+import time
+
+
+def _precision(hydro, r):
+    return hydro.FP32_FLUX if r.get("fp32flux") else hydro.FP64
+
+
+def percentile(xs, q: float) -> float:
+    """q-th percentile (0..100) with linear interpolation between order statistics."""
+    s = sorted(xs)
+    if not s:
+        return float("nan")
+    pos = (len(s) - 1) * q / 100.0
+    lo = int(pos)
+    hi = min(lo + 1, len(s) - 1)
+    return s[lo] + (s[hi] - s[lo]) * (pos - lo)
+
+
+def cell_updates(r) -> float:
+    c = 1
+    for n in r["nb"]:
+        c *= n
+    return float(c) * r["steps"]
+
+
+def run_trace(hydro, server, reqs, realtime: bool = True, keep: bool = False, progress=None) -> dict:
+    t_base = server.now()
+    ids = []
+    i, n, done = 0, len(reqs), 0
+    while done < n:
+        now = server.now() - t_base
+        while i < n and (not realtime or reqs[i]["arrival"] <= now):
+            r = reqs[i]
+            ids.append(server.submit(r["nb"], _precision(hydro, r), C=r["C"], radius_frac=r["radius_frac"],
+                                     p_in=r["p_in"], p_out=r["p_out"], rho_in=r["rho_in"], rho_out=r["rho_out"],
+                                     max_it=r["steps"], keep=keep,
+                                     arrival=t_base + (r["arrival"] if realtime else 0.0)))
+            i += 1
+        q, running, _ = server.pending()
+        if q == 0 and running == 0:
+            if i < n:
+                time.sleep(max(0.0, min(0.0005, reqs[i]["arrival"] - now)))
+            continue
+        k = server.tick()
+        done += k
+        if progress and k:
+            progress(done, n)
+    info = [server.request(rid) for rid in ids]
+    bad = [x for x in info if x["rc"] != 0 or x["state"] not in (server.DONE, server.RELEASED)]
+    if bad:
+        raise RuntimeError(f"served requests failed: {bad[:3]}")
+    lat = [x["done"] - x["arrival"] for x in info]
+    first = min(x["arrival"] for x in info)
+    last = max(x["done"] for x in info)
+    work = sum(cell_updates(r) for r in reqs)
+    return {"ids": ids, "info": info, "latency": lat, "makespan": last - first, "cell_updates": work,
+            "gcups": work / (last - first) / 1e9, "p50": percentile(lat, 50), "p99": percentile(lat, 99)}
+
+
+def plain_solve_times(hydro, torch, reqs, progress=None) -> list:
+    """Seconds per request solved alone on the device-resident single-solve loop (IC on
+    the device, then max_it = steps steps), one after another."""
+    out = []
+    for j, r in enumerate(reqs):
+        nb = r["nb"]
+        shape = tuple(x + 2 for x in nb)
+        a = [torch.empty(shape + (2,), dtype=torch.float64, device="cuda"),
+             torch.empty(shape + (len(nb),), dtype=torch.float64, device="cuda")]
+        b = [torch.empty_like(a[0]), torch.empty_like(a[1])]
+        k = hydro.constants(nb, r["C"])
+        dt0 = hydro.ic(a[0], a[1], k["C_min"], ic_mode=1, radius_frac=r["radius_frac"], p_in=r["p_in"],
+                       p_out=r["p_out"], rho_in=r["rho_in"], rho_out=r["rho_out"])
+        run = hydro.Run(nb, k["D"], k["C_min"], dt0, 1e30, r["steps"], _precision(hydro, r))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run.advance(a, b, r["steps"])
+        st = run.status()
+        out.append(time.perf_counter() - t0)
+        if st["rc"] != 0 or st["it"] != r["steps"]:
+            raise RuntimeError(f"plain solve of request {j} failed: {st}")
+        run.close()
+        del a, b
+        if progress:
+            progress(j + 1, len(reqs))
+    return out
+
+
+def sequential_latencies(reqs, durations) -> list:
+    """FIFO single server: start_i = max(arrival_i, finish_{i-1}); latency = finish - arrival."""
+    finish = 0.0
+    lat = []
+    for r, d in zip(reqs, durations):
+        start = max(r["arrival"], finish)
+        finish = start + d
+        lat.append(finish - r["arrival"])
+    return lat
