@@ -308,7 +308,7 @@ def main() -> int:
     ap.add_argument("--plain-sync-blasts", type=int, default=256)
     ap.add_argument("--requests", type=int, default=120, help="served workload: number of requests")
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0, help="served workload: state arena (GiB)")
-    ap.add_argument("--steps-per-tick", type=int, default=50)
+    ap.add_argument("--steps-per-tick", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()

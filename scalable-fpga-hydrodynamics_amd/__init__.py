@@ -116,6 +116,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_server_result.argtypes = [vp, i32, vp, vp]
     L.hydro_server_release.argtypes = [vp, i32]
     L.hydro_server_stats.argtypes = [vp, vp, vp]
+    L.hydro_server_set_limits.argtypes = [vp, d, sz]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
 
@@ -468,6 +469,10 @@ class Server:
 
     def release(self, rid: int) -> None:
         _check(lib().hydro_server_release(self._h, int(rid)), "hydro_server_release")
+
+    def set_limits(self, max_active_cells: float = 0.0, steps_per_tick: int = 50) -> None:
+        _check(lib().hydro_server_set_limits(self._h, float(max_active_cells), int(steps_per_tick)),
+               "hydro_server_set_limits")
 
     def stats(self):
         t, cu = C.c_size_t(), C.c_double()

@@ -140,7 +140,10 @@ __global__ void ghost_y2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long 
     M2[dst] = M2[src];
 }
 
-int g_rows_2d = 64;  // rows per strip (tunable, hydro_tune_set)
+int g_rows_2d = 64;  // maximum rows per strip (tunable, hydro_tune_set)
+// Workgroups a step should have to fill the chip (8 per CU on 256 CUs); smaller grids
+// use shorter strips (down to 8 rows) until they have that many.
+constexpr long long kFillBlocks2 = 2048;
 
 template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
@@ -163,9 +166,10 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     g.nb_x = static_cast<int>(nb_x);
     g.nb_y = static_cast<int>(nb_y);
     g.ys = static_cast<long long>(nb_y) + 2;
-    g.rows = g_rows_2d;
     g.nwin = static_cast<int>((nb_y + kOwned - 1) / kOwned);
     g.nwb = (g.nwin + kWavesPerBlock - 1) / kWavesPerBlock;
+    g.rows = g_rows_2d;
+    while (g.rows > 8 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nwb < kFillBlocks2) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
     const long long nblocks = nstrips * g.nwb;
     if (nblocks > 0x7fffffffLL) {

@@ -171,7 +171,10 @@ __global__ void ghost_z3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long
     M3[dst] = M3[src];
 }
 
-int g_rows_3d = 32;
+int g_rows_3d = 32;  // maximum X rows per strip (tunable, hydro_tune_set)
+// Workgroups a step should have to fill the chip (4 per CU); smaller grids use shorter
+// strips (down to 4 rows) until they have that many.
+constexpr long long kFillBlocks3 = 1024;
 
 template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
@@ -198,9 +201,10 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     g.nb_z = static_cast<int>(nb_z);
     g.zs = static_cast<long long>(nb_z) + 2;
     g.zy = (static_cast<long long>(nb_y) + 2) * g.zs;
-    g.rows = g_rows_3d;
     g.nzw = static_cast<int>((nb_z + kOwnedZ - 1) / kOwnedZ);
     g.nyw = static_cast<int>((nb_y + kTY - 1) / kTY);
+    g.rows = g_rows_3d;
+    while (g.rows > 4 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nzw * g.nyw < kFillBlocks3) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
     const long long nblocks = nstrips * g.nzw * g.nyw;
     if (nblocks > 0x7fffffffLL) {

@@ -119,8 +119,9 @@ __global__ void __launch_bounds__(kBlock, kBatchMinBlocks2)
     if (cb >= 0) flush_key(blasts, cb, key, wid, lane);
 }
 
+// Four waves per SIMD (two 8-wave workgroups per CU): at most 128 VGPRs.
 template <typename TS, typename TF, typename M>
-__global__ void __launch_bounds__(kBlock3)
+__global__ void __launch_bounds__(kBlock3, 4)
     batch3d_kernel(BatchBlast* __restrict__ blasts, const BatchSeg* __restrict__ segs,
                    const unsigned* __restrict__ seg_off, Consts<double> cs, unsigned nworkers)
 {

@@ -4,15 +4,18 @@
 // conditions) are queued on arrival and admitted in arrival order as soon as their
 // device footprint (two FP64 buffer pairs, ghosts included) fits in the server's HBM
 // arena and their engine has a free slot.  One batch engine per (dimension, precision)
-// advances all of its running requests together, each on its own stream, so requests of
-// every kind overlap on the device; a request joins its engine between ticks and leaves
-// it when its own loop ends (max_it or max_t, hydro2d/main.cxx:689), freeing memory for
-// the queue.  State stays on the device from the initial condition (written there) to
-// the end; a result is copied out only if the client asks for it.
+// advances all of its running requests together on its own stream, so requests of every
+// kind overlap on the device; a request joins its engine between that engine's ticks and
+// leaves it when its own loop ends (max_it or max_t, hydro2d/main.cxx:689), freeing
+// memory for the queue.  State stays on the device from the initial condition (written
+// there) to the end; a result is copied out only if the client asks for it.
 //
-// A tick: admit what fits, enqueue steps_per_tick steps on every engine with running
-// requests, wait, retire finished requests.  Latency = completion observed at the end
-// of a tick - arrival, on the server's steady clock.
+// Engines tick independently: each has an event marking the end of its last tick, and
+// hydro_server_tick serves every engine that is idle (retire its finished requests,
+// admit queued ones, enqueue its next steps_per_tick steps), waiting only when every
+// engine with work is still busy.  A small engine therefore never waits for a large
+// one.  Latency = completion observed after the request's last tick - arrival, on the
+// server's steady clock.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -20,6 +23,7 @@
 #include <map>
 #include <new>
 #include <string.h>
+#include <thread>
 #include <vector>
 
 #include "../../include/hydro_hip.h"
@@ -117,6 +121,8 @@ struct hydro_server {
     size_t capacity = 0;  // running requests per engine
     size_t steps_per_tick = 50;
     hydro_batch* eng[4] = { nullptr, nullptr, nullptr, nullptr };
+    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };  // end of the engine's last tick
+    bool busy[4] = { false, false, false, false };
     std::vector<int> slot_req[4];  // slot -> request id (-1 free)
     size_t running[4] = { 0, 0, 0, 0 };
     std::vector<Req> reqs;
@@ -124,6 +130,8 @@ struct hydro_server {
     std::chrono::steady_clock::time_point t0;
     size_t ticks = 0;
     double cell_updates = 0;  // of retired requests
+    double max_active_cells = 0;  // admission cap on interior cells in flight (0 = none)
+    double active_cells = 0;
 
     double now() const
     {
@@ -156,22 +164,36 @@ int ensure_engine(hydro_server* s, int e)
     s->eng[e] = hydro_batch_create(dim, prec, s->capacity);
     if (!s->eng[e]) return HYDRO_ERR_HIP;
     s->slot_req[e].assign(s->capacity, -1);
-    // Continuous batching replays a few steps at a time from a graph; tiles of 32 rows
-    // balance well across the changing mix.
+    if (hipEventCreateWithFlags(&s->ev[e], hipEventDisableTiming) != hipSuccess) {
+        set_error("hydro server: cannot create an event");
+        return HYDRO_ERR_HIP;
+    }
+    // Continuous batching replays a few steps at a time from a graph.  The balanced
+    // tiling (equal row ranges per worker) stays efficient whatever the mix, down to a
+    // single request; R-row tiles leave most workers idle when few requests run.
     int rc = hydro_batch_set_graph(s->eng[e], 10);
-    if (rc == 0) rc = hydro_batch_set_tiling(s->eng[e], 2, dim == 2 ? 32 : 0);
+    if (rc == 0) rc = hydro_batch_set_tiling(s->eng[e], dim == 2 ? 3 : 2, 0);
     return rc;
 }
 
-// Admit queued requests in arrival order while the head fits (memory and a slot).
+// Admit queued requests in arrival order.  A request whose engine is busy (mid-tick) or
+// full waits for it without holding back requests of other engines; admission stops at
+// the first request whose memory (or the active-cell cap) does not fit, so that later
+// requests never take the memory an earlier one is waiting for.
 int admit(hydro_server* s)
 {
-    while (!s->queue.empty()) {
-        Req& r = s->reqs[s->queue.front()];
+    for (auto q = s->queue.begin(); q != s->queue.end();) {
+        const int id = *q;
+        Req& r = s->reqs[id];
         const int e = engine_of(r);
         int rc = ensure_engine(s, e);
         if (rc) return rc;
-        if (s->running[e] >= s->capacity) break;
+        if (s->busy[e] || s->running[e] >= s->capacity) {
+            ++q;
+            continue;
+        }
+        const size_t n_running = s->running[0] + s->running[1] + s->running[2] + s->running[3];
+        if (s->max_active_cells > 0 && n_running > 0 && s->active_cells + r.interior > s->max_active_cells) break;
         size_t off = 0;
         if (!s->arena.alloc(r.bytes, off)) break;
         char* p = s->base + off;
@@ -190,9 +212,10 @@ int admit(hydro_server* s)
         r.slot = slot;
         r.state = kRunning;
         r.t_admit = s->now();
-        s->slot_req[e][slot] = s->queue.front();
+        s->slot_req[e][slot] = id;
         ++s->running[e];
-        s->queue.pop_front();
+        s->active_cells += r.interior;
+        q = s->queue.erase(q);
     }
     return 0;
 }
@@ -203,11 +226,11 @@ void release_memory(hydro_server* s, Req& r)
     r.state = r.state == kFailed ? kFailed : kReleased;
 }
 
-int retire(hydro_server* s, size_t* n_done)
+int retire(hydro_server* s, int e, size_t* n_done)
 {
     std::vector<unsigned char> flags(s->capacity);
-    for (int e = 0; e < 4; ++e) {
-        if (!s->eng[e] || s->running[e] == 0) continue;
+    {
+        if (!s->eng[e] || s->running[e] == 0) return 0;
         int rc = hydro_batch_done_flags(s->eng[e], flags.data(), s->capacity);
         if (rc) return rc;
         const double t = s->now();
@@ -224,6 +247,7 @@ int retire(hydro_server* s, size_t* n_done)
             if (rc) return rc;
             s->slot_req[e][slot] = -1;
             --s->running[e];
+            s->active_cells -= r.interior;
             if (!r.keep || r.state == kFailed) release_memory(s, r);
             if (n_done) ++*n_done;
         }
@@ -262,8 +286,10 @@ extern "C" hydro_server* hydro_server_create(size_t hbm_bytes, size_t capacity, 
 extern "C" void hydro_server_destroy(hydro_server* s)
 {
     if (!s) return;
-    for (int e = 0; e < 4; ++e)
+    for (int e = 0; e < 4; ++e) {
         if (s->eng[e]) hydro_batch_destroy(s->eng[e]);
+        if (s->ev[e]) (void)hipEventDestroy(s->ev[e]);
+    }
     if (s->base) (void)hipFree(s->base);
     delete s;
 }
@@ -321,15 +347,47 @@ extern "C" int hydro_server_tick(hydro_server* s, size_t* n_done)
 {
     if (!check(s, "hydro_server_tick")) return HYDRO_ERR_ARG;
     if (n_done) *n_done = 0;
-    int rc = admit(s);
-    if (rc) return rc;
-    for (int e = 0; e < 4; ++e)
-        if (s->eng[e] && s->running[e] > 0) {
+    // Wait until an engine that has work (running requests, or queued ones it could
+    // admit) is idle.
+    for (;;) {
+        bool any_busy = false, any_ready = false;
+        for (int e = 0; e < 4; ++e) {
+            if (s->busy[e]) {
+                const hipError_t q = hipEventQuery(s->ev[e]);
+                if (q == hipSuccess) s->busy[e] = false;
+                else if (q != hipErrorNotReady) {
+                    set_error("hydro_server_tick: %s", hipGetErrorString(q));
+                    return HYDRO_ERR_HIP;
+                }
+            }
+            any_busy = any_busy || s->busy[e];
+            any_ready = any_ready || (!s->busy[e] && s->running[e] > 0);
+        }
+        for (const int id : s->queue) {
+            const int e = engine_of(s->reqs[id]);
+            any_ready = any_ready || !s->busy[e];
+        }
+        if (any_ready || !any_busy) break;
+        std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    int rc = 0;
+    for (int e = 0; e < 4 && rc == 0; ++e)
+        if (!s->busy[e]) rc = retire(s, e, n_done);
+    if (rc == 0) rc = admit(s);
+    for (int e = 0; e < 4 && rc == 0; ++e)
+        if (s->eng[e] && !s->busy[e] && s->running[e] > 0) {
             rc = hydro_batch_advance(s->eng[e], s->steps_per_tick);
-            if (rc) return rc;
+            if (rc == 0) {
+                const hipError_t r = hipEventRecord(s->ev[e], static_cast<hipStream_t>(hydro_batch_stream(s->eng[e])));
+                if (r != hipSuccess) {
+                    set_error("hydro_server_tick: %s", hipGetErrorString(r));
+                    rc = HYDRO_ERR_HIP;
+                }
+                s->busy[e] = true;
+            }
         }
     ++s->ticks;
-    return retire(s, n_done);
+    return rc;
 }
 
 extern "C" int hydro_server_pending(hydro_server* s, size_t* queued, size_t* running, size_t* bytes_in_use)
@@ -385,6 +443,18 @@ extern "C" int hydro_server_release(hydro_server* s, int id)
         return HYDRO_ERR_ARG;
     }
     release_memory(s, s->reqs[id]);
+    return 0;
+}
+
+extern "C" int hydro_server_set_limits(hydro_server* s, double max_active_cells, size_t steps_per_tick)
+{
+    if (!check(s, "hydro_server_set_limits")) return HYDRO_ERR_ARG;
+    if (!(max_active_cells >= 0) || steps_per_tick < 1) {
+        set_error("hydro_server_set_limits: max_active_cells >= 0, steps_per_tick >= 1");
+        return HYDRO_ERR_ARG;
+    }
+    s->max_active_cells = max_active_cells;
+    s->steps_per_tick = steps_per_tick;
     return 0;
 }
 
