@@ -44,30 +44,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(dim: int, budget_s: float = 12.0) -> dict:
-    """The CPU oracle (a restatement of the reference step, OpenMP over X rows) timed on
-    this host on a bounded sample of the same blast: 4000 x 2000 in 2D, 128^3 in 3D."""
+def cpu_baseline(dim: int, budget_s: float = 14.0, serial_budget_s: float = 8.0) -> dict:
+    """The CPU oracle (a restatement of the reference step in its operation order) timed
+    on this host on a bounded sample of the same blast -- 4000 x 2000 in 2D, 128^3 in
+    3D -- two ways: OpenMP over X rows on the host's cores (at most 16), and serial (one
+    thread, the reference CPU build's single_task per rank).  `value` is the OpenMP
+    rate."""
     import numpy as np
 
     from oracle import oracle as O
 
-    threads = max(1, min(os.cpu_count() or 1, 16))
+    nproc = os.cpu_count() or 1
+    threads = max(1, min(nproc, 16))
     if dim == 2:
         nb = (4000, 2000)
         rhoE, mom, dt0 = O.ic2d(*nb)
-        run = lambda n: O.run2d(rhoE, mom, dt0, 2.0, n, nthreads=threads)  # noqa: E731
+        run = lambda n, th, st: O.run2d(st[0], st[1], dt0, 2.0, n, nthreads=th)  # noqa: E731
     else:
         nb = (128, 128, 128)
         rhoE, mom, dt0 = O.ic3d(*nb, ic_mode=1, radius_frac=0.1)
-        run = lambda n: O.run3d(rhoE, mom, dt0, 2.0, n, nthreads=threads)  # noqa: E731
+        run = lambda n, th, st: O.run3d(st[0], st[1], dt0, 2.0, n, nthreads=th)  # noqa: E731
     cells = int(np.prod(nb))
-    t0 = time.perf_counter()
-    run(2)
-    per_step = (time.perf_counter() - t0) / 2
-    steps = int(max(2, min(200, budget_s / max(per_step, 1e-6))))
-    t0 = time.perf_counter()
-    res = run(steps)
-    el = time.perf_counter() - t0
+
+    def timed(th, budget):
+        st = (rhoE.copy(), mom.copy())
+        t0 = time.perf_counter()
+        run(2, th, st)
+        per_step = (time.perf_counter() - t0) / 2
+        steps = int(max(2, min(2000, budget / max(per_step, 1e-6))))
+        st = (rhoE.copy(), mom.copy())
+        t0 = time.perf_counter()
+        res = run(steps, th, st)
+        el = time.perf_counter() - t0
+        return cells * res["it"] / el / 1e9, res["it"], el
+
+    par, par_it, par_s = timed(threads, budget_s)
+    ser, ser_it, ser_s = timed(1, serial_budget_s)
     cpu_model = ""
     try:
         for line in open("/proc/cpuinfo"):
@@ -77,12 +89,14 @@ def cpu_baseline(dim: int, budget_s: float = 12.0) -> dict:
     except OSError:
         pass
     return {
-        "value": cells * res["it"] / el / 1e9,
+        "value": round(par, 5),
         "unit": "GCUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"{'x'.join(map(str, nb))} blast, {res['it']} steps, FP64, {threads} OpenMP threads"
-                  f" ({el:.1f} s){' on ' + cpu_model if cpu_model else ''}",
+        "sample": f"{'x'.join(map(str, nb))} blast, FP64: {par_it} steps on {threads} OpenMP threads ({par_s:.1f} s); "
+                  f"host nproc {nproc}{', ' + cpu_model if cpu_model else ''}",
+        "serial": {"value": round(ser, 5), "unit": "GCUPS", "cores": 1,
+                   "sample": f"{ser_it} steps, 1 thread ({ser_s:.1f} s)"},
     }
 
 
@@ -125,9 +139,10 @@ def bench_batched(args, name: str) -> dict:
         batch.add_blast(a, b, p["C"], p["radius_frac"], p["p_in"], p["p_out"], p["rho_in"], p["rho_out"],
                         max_t=1e30, max_it=10**9)
     t0 = time.perf_counter()
-    tune_ms = batch.autotune(steps_per_trial=2, use_cache=False)
+    tune_ms = batch.autotune(steps_per_trial=4, use_cache=False)
     tuning = batch.tiling()
-    log(f"[bench] autotune {time.perf_counter() - t0:.2f} s: blocks/CU {tuning[0]}, tile rows {tuning[1]} "
+    log(f"[bench] autotune {time.perf_counter() - t0:.2f} s: blocks/CU {tuning[0]}, tile rows {tuning[1]}, "
+        f"tile windows {tuning[2]} "
         f"({tune_ms:.3f} ms/step)")
     graph_steps = 10
     batch.set_graph(graph_steps)
@@ -207,7 +222,8 @@ def bench_batched(args, name: str) -> dict:
         "dtype": "f64" if precision == 64 else "f64 state, f32 flux",
         "data": "synthetic (256 seeded blasts, splitmix64 seed 0x5EED, IC generated on the device)",
         "config": {"workload": name, "blasts": len(blasts), "cells": cells, "sides": "log-uniform [64, 2048]",
-                   "tiling": {"blocks_per_cu": tuning[0], "tile_rows": tuning[1]}, "graph_steps": graph_steps,
+                   "tiling": {"blocks_per_cu": tuning[0], "tile_rows": tuning[1], "tile_windows": tuning[2]},
+                   "graph_steps": graph_steps,
                    "kernel": "batch2d (persistent, one launch per step) + batch finalize"},
         "roofline": {"bound": "hbm", "achieved": round(64 * cells / per_step / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(64 * cells / per_step / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
@@ -304,6 +320,8 @@ def main() -> int:
     ap.add_argument("--rows", type=int, default=0, help="override the kernel's X rows per strip")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-reference", action="store_true", help="skip the achievable-bandwidth copy")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="default workload only: skip the batched and served summaries")
     ap.add_argument("--plain-steps", type=int, default=20, help="steps per blast of the one-solve-at-a-time baselines")
     ap.add_argument("--plain-sync-blasts", type=int, default=256)
     ap.add_argument("--requests", type=int, default=120, help="served workload: number of requests")
@@ -458,8 +476,26 @@ def main() -> int:
         "mass_change_per_cell": abs(mass1 - mass0) / cells,
         "t_final": st["t"],
     }
-    print(json.dumps(line), flush=True)
     run.close()
+    if not args.no_forms and name == "blast2d_40000x20000_fp64":
+        # The batched and served forms, summarised in the same line (each has its own
+        # workload for a full line: --workload batched / served).
+        a = b = cur = src = dst = None  # noqa: F841  (free the 51 GB state first)
+        torch.cuda.empty_cache()
+        bt = bench_batched(args, "batched2d_256_fp32flux")
+        torch.cuda.empty_cache()
+        sv = bench_served(args)
+        torch.cuda.empty_cache()
+        line["forms"] = {
+            "batched2d_256_fp32flux": {k: bt[k] for k in (
+                "value", "unit", "ms_per_step", "plain_resident_gcups", "plain_sync_launcher_gcups",
+                "speedup_vs_plain_resident", "speedup_vs_plain_sync_launcher", "plain_sample")}
+            | {"tiling": bt["config"]["tiling"], "roofline_achieved_gbs": bt["roofline"]["achieved"]},
+            "served_mixed": {"requests": sv["config"]["requests"], "arrival_rate_per_s": sv["config"]["arrival_rate_per_s"],
+                             "load": sv["config"]["load"], "served": sv["served"],
+                             "sequential_plain": sv["sequential_plain"], "saturation": sv["saturation"]},
+        }
+    print(json.dumps(line), flush=True)
     return 0
 
 

@@ -223,11 +223,13 @@ int hydro_batch_status(hydro_batch* batch, int slot, size_t* it, double* t, doub
  * non-finite Dt (waits for the batch stream). */
 int hydro_batch_done_flags(hydro_batch* batch, unsigned char* flags, size_t n);
 
-/* Tiling: workgroups per CU of the persistent grid (0 = default) and tile rows (0 =
- * balanced: equal contiguous row ranges per worker; R > 0: R-row tiles round-robin).
- * Results do not depend on the tiling. */
-int hydro_batch_set_tiling(hydro_batch* batch, int blocks_per_cu, int tile_rows);
-int hydro_batch_get_tiling(hydro_batch* batch, int* blocks_per_cu, int* tile_rows);
+/* Tiling: workgroups per CU of the persistent grid (0 = default), tile rows (0 =
+ * balanced: equal contiguous row ranges per worker; R > 0: R-row tiles dealt
+ * round-robin) and, in 2D, tile width in 62-column windows (1, 2 or 4 adjacent windows
+ * marched together by one worker of as many waves; 0 = 1).  Results do not depend on
+ * the tiling. */
+int hydro_batch_set_tiling(hydro_batch* batch, int blocks_per_cu, int tile_rows, int tile_windows);
+int hydro_batch_get_tiling(hydro_batch* batch, int* blocks_per_cu, int* tile_rows, int* tile_windows);
 
 /* Replay steps from a hipGraph of steps_per_graph steps (0 = direct launches). */
 int hydro_batch_set_graph(hydro_batch* batch, size_t steps_per_graph);

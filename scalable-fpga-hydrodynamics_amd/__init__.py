@@ -99,8 +99,8 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_batch_sync.argtypes = [vp]
     L.hydro_batch_status.argtypes = [vp, i32, vp, vp, vp, vp, vp]
     L.hydro_batch_done_flags.argtypes = [vp, vp, sz]
-    L.hydro_batch_set_tiling.argtypes = [vp, i32, i32]
-    L.hydro_batch_get_tiling.argtypes = [vp, vp, vp]
+    L.hydro_batch_set_tiling.argtypes = [vp, i32, i32, i32]
+    L.hydro_batch_get_tiling.argtypes = [vp, vp, vp, vp]
     L.hydro_batch_set_graph.argtypes = [vp, sz]
     L.hydro_batch_autotune.argtypes = [vp, sz, i32, vp]
     L.hydro_server_create.argtypes = [sz, sz, sz]
@@ -388,18 +388,21 @@ class Batch:
         _check(lib().hydro_batch_done_flags(self._h, out.ctypes.data_as(C.c_void_p), n), "hydro_batch_done_flags")
         return out
 
-    def set_tiling(self, blocks_per_cu: int = 0, tile_rows: int = 0) -> None:
-        _check(lib().hydro_batch_set_tiling(self._h, int(blocks_per_cu), int(tile_rows)), "hydro_batch_set_tiling")
+    def set_tiling(self, blocks_per_cu: int = 0, tile_rows: int = 0, tile_windows: int = 1) -> None:
+        _check(lib().hydro_batch_set_tiling(self._h, int(blocks_per_cu), int(tile_rows), int(tile_windows)),
+               "hydro_batch_set_tiling")
 
     def tiling(self):
-        bpc, rows = C.c_int(), C.c_int()
-        _check(lib().hydro_batch_get_tiling(self._h, C.byref(bpc), C.byref(rows)), "hydro_batch_get_tiling")
-        return bpc.value, rows.value
+        """(blocks per CU, tile rows, tile windows)."""
+        bpc, rows, width = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().hydro_batch_get_tiling(self._h, C.byref(bpc), C.byref(rows), C.byref(width)),
+               "hydro_batch_get_tiling")
+        return bpc.value, rows.value, width.value
 
     def set_graph(self, steps_per_graph: int) -> None:
         _check(lib().hydro_batch_set_graph(self._h, int(steps_per_graph)), "hydro_batch_set_graph")
 
-    def autotune(self, steps_per_trial: int = 2, use_cache: bool = True) -> float:
+    def autotune(self, steps_per_trial: int = 4, use_cache: bool = True) -> float:
         ms = C.c_double()
         _check(lib().hydro_batch_autotune(self._h, int(steps_per_trial), 1 if use_cache else 0, C.byref(ms)),
                "hydro_batch_autotune")

@@ -172,7 +172,7 @@ int ensure_engine(hydro_server* s, int e)
     // tiling (equal row ranges per worker) stays efficient whatever the mix, down to a
     // single request; R-row tiles leave most workers idle when few requests run.
     int rc = hydro_batch_set_graph(s->eng[e], 10);
-    if (rc == 0) rc = hydro_batch_set_tiling(s->eng[e], dim == 2 ? 3 : 2, 0);
+    if (rc == 0) rc = hydro_batch_set_tiling(s->eng[e], dim == 2 ? 3 : 2, 0, 1);
     return rc;
 }
 

@@ -84,13 +84,13 @@ def test_batch2d_matches_oracle_per_blast(gpu, oracle):
 
 
 def test_batch2d_tiling_changes_mid_run(gpu, oracle):
-    batch, slots = _make(gpu, 2, SIZES_2D, 24)
-    for bpc, rows in [(1, 0), (4, 32), (2, 128), (3, 0)]:
-        batch.set_tiling(bpc, rows)
-        assert batch.tiling() == (bpc, rows)
+    batch, slots = _make(gpu, 2, SIZES_2D, 30)
+    for bpc, rows, width in [(1, 0, 1), (4, 32, 2), (2, 128, 4), (3, 0, 4), (2, 0, 2)]:
+        batch.set_tiling(bpc, rows, width)
+        assert batch.tiling() == (bpc, rows, width)
         batch.advance(6)
     for slot, nb, C, rf, p_in, mi in slots:
-        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 24), f"2D tiling {nb}")
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 30), f"2D tiling {nb}")
     batch.close()
 
 
@@ -113,15 +113,16 @@ def test_batch2d_autotune_steps_are_solve_steps(gpu, oracle):
     batch, slots = _make(gpu, 2, SIZES_2D, 40)
     ms = batch.autotune(steps_per_trial=1, use_cache=False)
     assert ms > 0
-    bpc, rows = batch.tiling()
-    ntrials = 4 * 4  # blocks per CU 1..4 x (balanced, 32, 64, 128)
-    batch.advance(30 - ntrials)
+    tiling = batch.tiling()
+    # blocks per CU 2..4 x rows (balanced, 32, 128) x width (1, 2, 4), then the best 3 again
+    ntrials = 3 * 3 * 3 + 3
+    batch.advance(40 - ntrials)
     for slot, nb, C, rf, p_in, mi in slots:
-        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 30), f"2D tuned ({bpc},{rows}) {nb}")
+        _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 40), f"2D tuned {tiling} {nb}")
     # A second batch of the same signature reuses the tuned tiling without trials.
     again, _ = _make(gpu, 2, SIZES_2D, 40)
     assert again.autotune(steps_per_trial=1, use_cache=True) == -1.0
-    assert again.tiling() == (bpc, rows)
+    assert again.tiling() == tiling
     again.close()
     batch.close()
 
