@@ -1,0 +1,118 @@
+"""The benchmark configurations at their real sizes, through properties that do not depend
+on size, beside bit-exact oracle comparisons at reduced sizes.
+
+2D: 40000 x 20000 FP64 blast (reference IC scaled to the grid), 100 steps: the blast is
+mirror-symmetric in X and Y about the grid centre (hydro2d/main.cxx:288-310) and the
+scheme preserves that exactly (negation is exact and mirrored cells see mirrored
+operands in the same order), so every field must be bitwise symmetric (up to the sign of
+zero) after any number of steps, and mass must be conserved to round-off (X periodic,
+zero wall mass flux).  The same run at 4000 x 2000 must equal the oracle bit for bit.
+
+3D: 512^3 blast with the parameterised IC (radius 0.1 L, pressure ratio 100), 200 steps:
+the FP32-flux form against FP64 per field (relative L-inf <= 1e-5, scales as in
+fields.py), symmetry of the FP64 result in X, Y and Z, and 128^3 of the same IC equal to
+the oracle bit for bit.
+"""
+import numpy as np
+import pytest
+import torch
+
+from fields import field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def _state(nb):
+    shape = tuple(n + 2 for n in nb)
+    a = [torch.zeros(shape + (2,), dtype=torch.float64, device="cuda"),
+         torch.zeros(shape + (len(nb),), dtype=torch.float64, device="cuda")]
+    b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+    return a, b
+
+
+def _run(gpu, nb, C, steps, precision, ic_mode=0, radius_frac=0.0, p_in=10.0):
+    a, b = _state(nb)
+    k = gpu.constants(nb, C)
+    dt0 = gpu.ic(a[0], a[1], k["C_min"], ic_mode=ic_mode, radius_frac=radius_frac, p_in=p_in, p_out=0.1)
+    m0 = gpu.total_mass(a[0])
+    run = gpu.Run(nb, k["D"], k["C_min"], dt0, 1e30, steps, precision)
+    run.advance(a, b, steps)
+    st = run.status()
+    run.close()
+    assert st["rc"] == 0 and st["it"] == steps, st
+    cur = a if st["current"] == 0 else b
+    other = b if st["current"] == 0 else a
+    del other
+    m1 = gpu.total_mass(cur[0])
+    return cur, st, m0, m1
+
+
+def _assert_mirror(rhoE, mom, dim, tag):
+    """Interior fields symmetric under i -> N+1-i on every axis; the momentum component
+    normal to the mirror changes sign."""
+    sl = (slice(1, -1),) * dim
+    r = rhoE[sl]
+    m = mom[sl]
+    for ax in range(dim):
+        fr = torch.flip(r, dims=[ax])
+        assert torch.equal(fr, r), f"{tag}: rho/E not symmetric along axis {ax}"
+        del fr
+        fm = torch.flip(m, dims=[ax])
+        sign = torch.ones(dim, dtype=torch.float64, device="cuda")
+        sign[ax] = -1.0
+        # IEEE equality: +0 == -0, so this is symmetry up to the sign of zero.
+        assert torch.equal(fm * sign, m), f"{tag}: momentum not (anti)symmetric along axis {ax}"
+        del fm
+        torch.cuda.empty_cache()
+
+
+def test_c2_full_size_symmetry_and_mass(gpu):
+    nb = (40000, 20000)
+    cur, st, m0, m1 = _run(gpu, nb, 0.5, 100, gpu.FP64)
+    print(f"[c2 40000x20000] t={st['t']:.6e} mass {m0:.10e} -> {m1:.10e}", flush=True)
+    assert abs(m1 - m0) / (nb[0] * nb[1]) < 1e-13
+    _assert_mirror(cur[0], cur[1], 2, "40000x20000")
+    assert torch.isfinite(cur[0][1:-1, 1:-1]).all()
+
+
+def test_c2_reduced_matches_oracle(gpu, oracle):
+    nb = (4000, 2000)
+    cur, st, _, _ = _run(gpu, nb, 0.5, 100, gpu.FP64)
+    rhoE, mom, dt0 = oracle.ic2d(*nb)
+    res = oracle.run2d(rhoE, mom, dt0, 1e30, 100, nthreads=16)
+    assert res["it"] == st["it"] and res["t"] == st["t"] and res["dt"] == st["dt"]
+    sl = (slice(1, -1),) * 2
+    np.testing.assert_array_equal(cur[0].cpu().numpy()[sl].view(np.uint64), rhoE[sl].view(np.uint64))
+    np.testing.assert_array_equal(cur[1].cpu().numpy()[sl].view(np.uint64), mom[sl].view(np.uint64))
+
+
+def test_c3_fp32flux_vs_fp64_full_size(gpu):
+    nb = (512, 512, 512)
+    ref, st_ref, m0, m1 = _run(gpu, nb, 0.4, 200, gpu.FP64, ic_mode=1, radius_frac=0.1)
+    assert abs(m1 - m0) / float(np.prod(nb)) < 1e-13
+    _assert_mirror(ref[0], ref[1], 3, "512^3 fp64")
+    ref_np = (ref[0].cpu().numpy(), ref[1].cpu().numpy())
+    del ref
+    torch.cuda.empty_cache()
+    got, st_got, g0, g1 = _run(gpu, nb, 0.4, 200, gpu.FP32_FLUX, ic_mode=1, radius_frac=0.1)
+    got_np = (got[0].cpu().numpy(), got[1].cpu().numpy())
+    del got
+    torch.cuda.empty_cache()
+    errs = field_errors(got_np, ref_np)
+    print("[c3 512^3 x 200, fp32-flux vs fp64] " + ", ".join(f"{k}: Linf {v[0]:.2e} L2 {v[1]:.2e}"
+                                                           for k, v in errs.items())
+          + f"; t {st_got['t']:.9e} vs {st_ref['t']:.9e}; mass drift {abs(g1 - g0) / g0:.1e}", flush=True)
+    for k, (linf, _) in errs.items():
+        assert linf <= 1e-5, (k, linf)
+    assert abs(g1 - g0) / float(np.prod(nb)) < 1e-13
+
+
+def test_c3_reduced_matches_oracle(gpu, oracle):
+    nb = (128, 128, 128)
+    cur, st, _, _ = _run(gpu, nb, 0.4, 200, gpu.FP64, ic_mode=1, radius_frac=0.1)
+    rhoE, mom, dt0 = oracle.ic3d(*nb, ic_mode=1, radius_frac=0.1)
+    res = oracle.run3d(rhoE, mom, dt0, 1e30, 200, nthreads=16)
+    assert res["it"] == st["it"] and res["t"] == st["t"] and res["dt"] == st["dt"]
+    sl = (slice(1, -1),) * 3
+    np.testing.assert_array_equal(cur[0].cpu().numpy()[sl].view(np.uint64), rhoE[sl].view(np.uint64))
+    np.testing.assert_array_equal(cur[1].cpu().numpy()[sl].view(np.uint64), mom[sl].view(np.uint64))
