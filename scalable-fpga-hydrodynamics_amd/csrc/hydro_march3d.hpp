@@ -22,7 +22,19 @@
 namespace hydro {
 
 constexpr int kOwnedZ = kWave - 2;
-constexpr int kTY = 6;               // owned Y rows per workgroup
+#ifndef HYDRO_3D_TY
+#define HYDRO_3D_TY 6
+#endif
+// Rows in flight ahead of row i+1 (register prefetch ring depth).
+#ifndef HYDRO_3D_PREFETCH
+#define HYDRO_3D_PREFETCH 1
+#endif
+// Minimum waves per SIMD of the 3D step kernel (4: two 8-wave workgroups per CU).
+#ifndef HYDRO_3D_MIN_WAVES
+#define HYDRO_3D_MIN_WAVES 4
+#endif
+constexpr int kTY = HYDRO_3D_TY;     // owned Y rows per workgroup
+constexpr int kPF3 = HYDRO_3D_PREFETCH;
 constexpr int kNW = kTY + 2;         // waves per workgroup
 constexpr int kBlock3 = kNW * kWave;
 
@@ -126,12 +138,16 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
     using V2 = typename Vec2_3<TS>::type;
     auto& s_prim = lds.prim;
     auto& s_flux = lds.flux;
-    V2 rc, rn;
-    Trip<TS> mc, mn;
+    // Row i in rc/mc; rows i+1 .. i+kPF3 in the ring qr/qm (slot kPF3 takes the next
+    // load each step).
+    V2 rc, qr[kPF3 + 1];
+    Trip<TS> mc, qm[kPF3 + 1];
     P6<TF> Pc;
     TF fx1_0 = TF(0), fx1_1 = TF(0), fx1_2 = TF(0), fx1_3 = TF(0), fx1_4 = TF(0);
     load3<TS>(R2, M3, row3_of<WRAP_X>(i0, nb_x) * zy + col, rc, mc);
-    load3<TS>(R2, M3, row3_of<WRAP_X>(i0 + 1, nb_x) * zy + col, rn, mn);
+#pragma unroll
+    for (int q = 0; q < kPF3; ++q)
+        load3<TS>(R2, M3, row3_of<WRAP_X>(min(i0 + 1 + q, i1 + 1), nb_x) * zy + col, qr[q], qm[q]);
     to_prim<TS, TF, M>(rc, mc, gmo, gamma, Pc);
     if (!halo_wave) {
         V2 ra;
@@ -143,10 +159,10 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
     }
 
     for (int i = i0; i <= i1; ++i) {
-        V2 rnn;
-        Trip<TS> mnn;
-        const int ipf = min(i + 2, i1 + 1);
-        load3<TS>(R2, M3, row3_of<WRAP_X>(ipf, nb_x) * zy + col, rnn, mnn);
+        const int ipf = min(i + 1 + kPF3, i1 + 1);
+        load3<TS>(R2, M3, row3_of<WRAP_X>(ipf, nb_x) * zy + col, qr[kPF3], qm[kPF3]);
+        const V2 rn = qr[0];
+        const Trip<TS> mn = qm[0];
 
         // (a) publish this row's Y-frame primitives.
         const Prim3<TF> Sy = frame_y(Pc);
@@ -269,8 +285,11 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
         Pc = Pn;
         rc = rn;
         mc = mn;
-        rn = rnn;
-        mn = mnn;
+#pragma unroll
+        for (int q = 0; q < kPF3; ++q) {
+            qr[q] = qr[q + 1];
+            qm[q] = qm[q + 1];
+        }
     }
 
     return key;
