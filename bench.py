@@ -311,6 +311,35 @@ def bench_served(args) -> dict:
     return line
 
 
+def bench_csv(args) -> dict:
+    """The CSV output path of the command-line driver at scale: the same run with and
+    without output; the difference is the cost of writing one final file of nb_x * nb_y
+    rows (byte-identical to the reference's std::ostream formatting)."""
+    import subprocess
+    import tempfile
+
+    exe = ROOT / "scalable-fpga-hydrodynamics_amd" / "hydro"
+    nx, ny = args.csv_grid
+    base = [str(exe), "-x", str(nx), "-y", str(ny), "-i", "2", "-t", "2"]
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        t0 = time.perf_counter()
+        subprocess.run(base + ["--no-output"], cwd=d, check=True, capture_output=True)
+        t_no = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        subprocess.run(base + ["-o", "out"], cwd=d, check=True, capture_output=True)
+        t_out = time.perf_counter() - t0
+        files = sorted(pathlib.Path(d).glob("out_*.csv"))
+        nbytes = sum(f.stat().st_size for f in files)
+    rows = nx * ny * len(files)
+    extra = max(t_out - t_no, 1e-9)
+    log(f"[bench] csv: {len(files)} files, {rows / 1e6:.1f} M rows, {nbytes / 1e9:.2f} GB in {extra:.2f} s extra")
+    return {"metric": "CSV rows/s (driver output path)", "value": round(rows / extra, 1), "unit": "rows/s",
+            "n_gpus": 1, "higher_is_better": True, "config": {"workload": f"csv_{nx}x{ny}", "files": len(files),
+                                                             "rows": rows, "bytes": nbytes},
+            "gb_per_s": round(nbytes / extra / 1e9, 3), "seconds_with_output": round(t_out, 3),
+            "seconds_without_output": round(t_no, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +356,7 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=120, help="served workload: number of requests")
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0, help="served workload: state arena (GiB)")
     ap.add_argument("--steps-per-tick", type=int, default=20)
+    ap.add_argument("--csv-grid", type=int, nargs=2, default=[6000, 5000], help="csv workload: grid of the driver run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -337,6 +367,9 @@ def main() -> int:
         return 2
 
     name = ALIASES.get(args.workload, args.workload)
+    if name == "csv":
+        print(json.dumps(bench_csv(args)), flush=True)
+        return 0
     if name == "served":
         import torch
 

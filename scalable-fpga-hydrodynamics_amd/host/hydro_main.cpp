@@ -23,6 +23,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <string>
 #include <thread>
@@ -119,9 +120,64 @@ int parse_bc(const char* s)
     exit(1);
 }
 
+// One background file writer: start() runs a job on its own thread; join() waits for it
+// and exits the program if the job failed (the reference aborts on an output error too).
+class Writer {
+public:
+    template <typename F>
+    void start(F job, const std::string& what)
+    {
+        join();
+        what_ = what;
+        failed_ = false;
+        th_ = std::thread([this, job] { failed_ = !job(); });
+    }
+    void join()
+    {
+        if (!th_.joinable()) return;
+        th_.join();
+        if (failed_) {
+            fprintf(stderr, "Error: cannot write %s\n", what_.c_str());
+            exit(1);
+        }
+    }
+    ~Writer()
+    {
+        if (th_.joinable()) th_.join();
+    }
+
+private:
+    std::thread th_;
+    std::atomic<bool> failed_{ false };
+    std::string what_;
+};
+
+template <typename T>
+struct HostCopy {
+    T* re = nullptr;
+    T* mo = nullptr;
+    Writer writer;  // declared after the buffers: destroyed (joined) before they are freed
+    void alloc(size_t n_re, size_t n_mo)
+    {
+        if (re) return;
+        if (hipHostMalloc(reinterpret_cast<void**>(&re), n_re * sizeof(T), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(reinterpret_cast<void**>(&mo), n_mo * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+            fprintf(stderr, "Error: cannot allocate %.2f GB of pinned host memory\n",
+                    static_cast<double>((n_re + n_mo) * sizeof(T)) / 1e9);
+            exit(1);
+        }
+    }
+    ~HostCopy()
+    {
+        writer.join();
+        if (re) (void)hipHostFree(re);
+        if (mo) (void)hipHostFree(mo);
+    }
+};
+
 // Compensated long double sum of interior rho, hydro2d/main.cxx:237-265.
 template <typename T>
-double total_mass(const std::vector<T>& rhoE, const size_t* nb)
+double total_mass(const T* rhoE, const size_t* nb)
 {
     long double tm = 0.0L, c = 0.0L;
     const size_t nz = kDim == 3 ? nb[2] : 1;
@@ -258,12 +314,16 @@ int run(const Options& o, const char* argv0)
     }
     fprintf(stderr, "done\n");
 
-    std::vector<T> h_re, h_mo;  // host copies, allocated on first use
+    // Host copy of the state for the CSV files and the host mass sum: pinned memory
+    // (hipHostMalloc), allocated on first use.  A file is formatted and written by a
+    // background thread from this copy while the device goes on stepping; the next
+    // fetch (and the end of the run) waits for that writer first.
+    HostCopy<T> host;
     auto fetch = [&](int cur) {
-        h_re.resize(n_re);
-        h_mo.resize(n_mo);
-        die_hip(hipMemcpy(h_re.data(), d_re[cur], n_re * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        die_hip(hipMemcpy(h_mo.data(), d_mo[cur], n_mo * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        host.writer.join();
+        host.alloc(n_re, n_mo);
+        die_hip(hipMemcpy(host.re, d_re[cur], n_re * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        die_hip(hipMemcpy(host.mo, d_mo[cur], n_mo * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     };
     auto device_mass = [&](int cur) {
         double m = 0;
@@ -282,10 +342,9 @@ int run(const Options& o, const char* argv0)
     auto write_csv = [&](size_t it, double t) {
         if (o.no_output) return;
         const std::string fname = o.out + "_" + std::to_string(it) + "_" + hydro_io::format_fixed4(t) + ".csv";
-        if (!hydro_io::write_csv<T>(fname, h_re.data(), h_mo.data(), kDim, nb, o.threads)) {
-            fprintf(stderr, "Error: cannot write %s\n", fname.c_str());
-            exit(1);
-        }
+        host.writer.start([fname, &host, &nb, &o] {
+            return hydro_io::write_csv<T>(fname, host.re, host.mo, kDim, nb, o.threads);
+        }, fname);
     };
 
     // Initial mass: host compensated sum as the reference (hydro2d/main.cxx:662), or the
@@ -295,7 +354,7 @@ int run(const Options& o, const char* argv0)
         initial_mass = device_mass(0);
     } else {
         fetch(0);
-        initial_mass = total_mass<T>(h_re, nb);
+        initial_mass = total_mass<T>(host.re, nb);
     }
     fprintf(stderr, "\nInit conditions : Diffusion of a 100 times higher pressurized circle at simulation center\n");
     fprintf(stderr, "  Default density      : %.2f inside, %.2f outside\n", static_cast<double>(rho_in),
@@ -383,7 +442,7 @@ int run(const Options& o, const char* argv0)
             T_.total.push_back(now_us() - u0);
             if (want_write(it, t)) {
                 fetch(cur);
-                mass_line(initial_mass, total_mass<T>(h_re, nb), it, t);
+                mass_line(initial_mass, total_mass<T>(host.re, nb), it, t);
                 write_csv(it, t);
             }
             it_line(it, t, static_cast<double>(Dt), T_.total.back(), us);
@@ -430,7 +489,7 @@ int run(const Options& o, const char* argv0)
                 T_.compute.push_back(comp_us);
                 if (i + 1 == it_now && want_write(i, tt[i])) {
                     fetch(curb);
-                    mass_line(initial_mass, total_mass<T>(h_re, nb), i, tt[i]);
+                    mass_line(initial_mass, total_mass<T>(host.re, nb), i, tt[i]);
                     write_csv(i, tt[i]);
                 }
                 it_line(i, tt[i], td[i], per_step_us, comp_us);
@@ -453,9 +512,10 @@ int run(const Options& o, const char* argv0)
     const double simu_t2 = now_us();
 
     if (!o.no_output) fetch(cur);
-    const double final_mass = o.no_output ? device_mass(cur) : total_mass<T>(h_re, nb);
+    const double final_mass = o.no_output ? device_mass(cur) : total_mass<T>(host.re, nb);
     const double final_change = mass_line(initial_mass, final_mass, it, t);
     write_csv(it, t);
+    host.writer.join();
 
     const double t_simu = (simu_t2 - simu_t1) / 1e6;
     printf("\n");
