@@ -1,0 +1,81 @@
+"""The drivers' CSV writer (host/hydro_io.cpp) against printf's %g, the formatting
+std::ostream applies by default (hydro2d/main.cxx:166-235): a harness built with g++
+writes the same random states (special values included) through hydro_io::write_csv and
+through snprintf, and the files must be byte-identical, for 2D and 3D, several 131072-row
+blocks per file, one and many formatting threads."""
+import pathlib
+import subprocess
+
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+HOST = ROOT / "scalable-fpga-hydrodynamics_amd" / "host"
+
+HARNESS = r"""
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+#include "hydro_io.hpp"
+
+int main(int argc, char** argv) {
+    const int dim = atoi(argv[1]);
+    size_t nb[3] = { (size_t)atoll(argv[2]), (size_t)atoll(argv[3]), dim == 3 ? (size_t)atoll(argv[4]) : 1 };
+    const int threads = atoi(argv[5]);
+    const size_t nz = dim == 3 ? nb[2] + 2 : 1;
+    const size_t cells = (nb[0] + 2) * (nb[1] + 2) * nz;
+    std::vector<double> re(2 * cells), mo(dim * cells);
+    std::mt19937_64 rng(42);
+    const double specials[] = { 0.0, -0.0, 1e-5, 9.999995e-5, 123456.5, 999999.5, 1e300, 5e-324, NAN, -NAN,
+                                INFINITY, -INFINITY, 1.2, 0.25, 1e6 };
+    auto draw = [&]() {
+        const uint64_t u = rng();
+        if (u % 17 == 0) return specials[(u >> 8) % (sizeof(specials) / sizeof(double))];
+        if (u % 5 == 0) { double v; uint64_t w = rng(); memcpy(&v, &w, 8); return v; }
+        return std::ldexp((double)(rng() >> 11) * 0x1p-53, (int)(rng() % 60) - 30) * ((u & 2) ? -1.0 : 1.0);
+    };
+    for (auto& v : re) v = draw();
+    for (auto& v : mo) v = draw();
+    if (!hydro_io::write_csv<double>("got.csv", re.data(), mo.data(), dim, nb, threads)) return 2;
+    FILE* f = fopen("want.csv", "wb");
+    fputs(dim == 2 ? "x,y,rho,p,u,v\n" : "x,y,z,rho,p,u,v,w\n", f);
+    const size_t ys = nb[1] + 2;
+    for (size_t i = 0; i < nb[0]; ++i)
+        for (size_t j = 0; j < nb[1]; ++j)
+            for (size_t k = 0; k < (dim == 3 ? nb[2] : 1); ++k) {
+                if (dim == 2) {
+                    const size_t n = (i + 1) * ys + (j + 1);
+                    fprintf(f, "%zu,%zu,%g,%g,%g,%g\n", i, j, re[2 * n], re[2 * n + 1], mo[2 * n], mo[2 * n + 1]);
+                } else {
+                    const size_t n = ((i + 1) * ys + (j + 1)) * nz + (k + 1);
+                    fprintf(f, "%zu,%zu,%zu,%g,%g,%g,%g,%g\n", i, j, k, re[2 * n], re[2 * n + 1], mo[3 * n],
+                            mo[3 * n + 1], mo[3 * n + 2]);
+                }
+            }
+    fclose(f);
+    return 0;
+}
+"""
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    d = tmp_path_factory.mktemp("csv")
+    src = d / "h.cpp"
+    src.write_text(HARNESS)
+    exe = d / "h"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", f"-I{HOST}", str(src), str(HOST / "hydro_io.cpp"), "-o",
+                    str(exe)], check=True, capture_output=True)
+    return exe
+
+
+@pytest.mark.parametrize("dim,nb,threads", [(2, (300, 500, 1), 4), (2, (7, 3, 1), 1), (3, (40, 30, 120), 16),
+                                            (3, (1, 1, 1), 0)])
+def test_csv_writer_matches_printf(harness, tmp_path, dim, nb, threads):
+    subprocess.run([str(harness), str(dim), *map(str, nb), str(threads)], cwd=tmp_path, check=True, timeout=120)
+    got = (tmp_path / "got.csv").read_bytes()
+    want = (tmp_path / "want.csv").read_bytes()
+    assert got.count(b"\n") == nb[0] * nb[1] * (nb[2] if dim == 3 else 1) + 1
+    assert got == want
