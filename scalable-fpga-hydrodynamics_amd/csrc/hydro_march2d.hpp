@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <utility>
+
 #include "hydro_device.hpp"
 
 namespace hydro {
@@ -30,6 +32,9 @@ constexpr int kWavesPerBlock = 4;
 #define HYDRO_2D_PREFETCH 3
 #endif
 constexpr int kPF = HYDRO_2D_PREFETCH;
+#ifndef HYDRO_2D_UNROLL
+#define HYDRO_2D_UNROLL 0
+#endif
 // Non-temporal (streaming) stores of the new state.
 #ifndef HYDRO_2D_NT_STORE
 #define HYDRO_2D_NT_STORE 0
@@ -122,9 +127,87 @@ __device__ __forceinline__ void window_lane2d(int j, int nb_y, int& jl, unsigned
     }
 }
 
+// One row of the march: the Y face of row i (lane j, handed to lane j+1), the primitives
+// of row i+1 (into Pn) and the X face (i, i+1) (into F2), the update of row i with the
+// X face (i-1, i) in F1, its store and the order key of its new signal speed.
+template <typename TS, typename TF, typename M>
+__device__ __forceinline__ uint64_t row2d(const typename Vec2<TS>::type& rc, const typename Vec2<TS>::type& mc,
+                                          const typename Vec2<TS>::type& rn, const typename Vec2<TS>::type& mn,
+                                          const Prim2<TF>& Pc, Prim2<TF>& Pn, const TF (&F1)[4], TF (&F2)[4],
+                                          typename Vec2<TS>::type* __restrict__ RN2,
+                                          typename Vec2<TS>::type* __restrict__ MN2, long long ys, int i, int j,
+                                          unsigned int ymask, bool owned, TF gamma, TF gmo, TF divgamma, TF K,
+                                          TS DtDx, TS DtDy, uint64_t key)
+{
+    // Y face (j, j+1) of row i in the (rho, v, u, p) frame (kernel.cxx:213-251),
+    // evaluated by lane j and handed to lane j+1.
+    Prim2<TF> Sy, Ny;
+    Sy.r = Pc.r;
+    Sy.un = xor_sign(Pc.ut, ymask);
+    Sy.ut = Pc.un;
+    Sy.p = Pc.p;
+    Sy.a = Pc.a;
+    Ny.r = from_next_lane(Sy.r);
+    Ny.un = from_next_lane(Sy.un);
+    Ny.ut = from_next_lane(Sy.ut);
+    Ny.p = from_next_lane(Sy.p);
+    Ny.a = from_next_lane(Sy.a);
+    TF fyR0, fyR1, fyR2, fyR3;
+    face_flux2<TF, M>(Sy, Ny, K, divgamma, fyR0, fyR1, fyR2, fyR3);
+    const TF fyL0 = from_prev_lane(fyR0);
+    const TF fyL1 = from_prev_lane(fyR1);
+    const TF fyL2 = from_prev_lane(fyR2);
+    const TF fyL3 = from_prev_lane(fyR3);
+
+    // Row i+1 primitives and the X face (i, i+1).
+    Pn.r = static_cast<TF>(rn.x);
+    cons_to_prim2<TF, M>(Pn.r, static_cast<TF>(mn.x), static_cast<TF>(mn.y), static_cast<TF>(rn.y), gmo, gamma, Pn.un,
+                         Pn.ut, Pn.p, Pn.a);
+    face_flux2<TF, M>(Pc, Pn, K, divgamma, F2[0], F2[1], F2[2], F2[3]);
+
+    // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the (rho, v, u, E)
+    // frame, so rho*u takes component 2 and rho*v component 1.  Increments and update
+    // are in the state precision TS.
+    const TS x_r = DtDx * (static_cast<TS>(F1[0]) - static_cast<TS>(F2[0]));
+    const TS x_u = DtDx * (static_cast<TS>(F1[1]) - static_cast<TS>(F2[1]));
+    const TS x_v = DtDx * (static_cast<TS>(F1[2]) - static_cast<TS>(F2[2]));
+    const TS x_E = DtDx * (static_cast<TS>(F1[3]) - static_cast<TS>(F2[3]));
+    const TS y_r = DtDy * (static_cast<TS>(fyL0) - static_cast<TS>(fyR0));
+    const TS y_u = DtDy * (static_cast<TS>(fyL2) - static_cast<TS>(fyR2));
+    const TS y_v = DtDy * (static_cast<TS>(fyL1) - static_cast<TS>(fyR1));
+    const TS y_E = DtDy * (static_cast<TS>(fyL3) - static_cast<TS>(fyR3));
+    const TS nr = rc.x + x_r + y_r;
+    const TS nu = mc.x + x_u + y_u;
+    const TS nv = mc.y + x_v + y_v;
+    const TS nE = rc.y + x_E + y_E;
+    const TF sp = signal_speed2<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
+                                       static_cast<TF>(nE), gmo, gamma);
+    const uint64_t k_new = speed_key(static_cast<double>(sp));
+    key = owned ? umax64(key, k_new) : key;
+    if (owned) {
+        const long long n = static_cast<long long>(i) * ys + j;
+        store_pair<TS>(RN2 + n, nr, nE);
+        store_pair<TS>(MN2 + n, nu, nv);
+    }
+    return key;
+}
+
+template <typename F, int... P>
+__device__ __forceinline__ bool run_phases(F& step, std::integer_sequence<int, P...>)
+{
+    return (step(std::integral_constant<int, P>{}) && ...);
+}
+
 // March rows i0..i1 (i0 <= i1) of one window.  Lane columns: j (global, for stores and
 // ownership) and jl (the column loaded).  Returns `key` folded with the order keys of
 // the new signal speeds of the owned cells.
+//
+// kPF rows of loads are in flight while a row is computed.  HYDRO_2D_UNROLL: rows
+// i .. i+1+kPF live in a ring of kPF+2 register slots and the primitives and X-face
+// flux of rows i and i+1 in two slots each, and the loop is unrolled over the phases of
+// both rings (lcm(kPF+2, 2) rows), so that slot indices are compile-time constants and
+// advancing the march moves no registers (at the price of register pressure); otherwise
+// the rolled loop shifts the ring by register moves.
 template <typename TS, typename TF, typename M, bool WRAP_X>
 __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __restrict__ R2,
                                             const typename Vec2<TS>::type* __restrict__ M2,
@@ -133,10 +216,43 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
                                             int i1, int j, int jl, unsigned int ymask, bool owned, TF gamma, TF gmo,
                                             TF divgamma, TF K, TS DtDx, TS DtDy, uint64_t key)
 {
+#if HYDRO_2D_UNROLL
     using V2 = typename Vec2<TS>::type;
-    // Rows i0-1 and i0; X face (i0-1, i0).  Rows i0+1 .. i0+kPF are loaded ahead into
-    // a register ring (qr/qm), so that kPF rows per wave are in flight while a row is
-    // computed.
+    constexpr int S = kPF + 2;                  // row slots: row r in slot (r - i0) % S
+    constexpr int U = (S % 2 == 0) ? S : 2 * S;  // phases of the row ring and the 2-slot rings
+    V2 ra, ma, qr[S], qm[S];
+    load_cell<TS>(R2, M2, row_of<WRAP_X>(i0 - 1, nb_x) * ys + jl, ra, ma);
+#pragma unroll
+    for (int q = 0; q <= kPF; ++q)
+        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i0 + q, i1 + 1), nb_x) * ys + jl, qr[q], qm[q]);
+    Prim2<TF> P[2];  // X frame: un = u, ut = v; P[(i - i0) % 2] is row i
+    TF F[2][4];      // F[(i - i0) % 2] is the X face (i-1, i)
+    {
+        Prim2<TF> Pm;
+        Pm.r = static_cast<TF>(ra.x);
+        cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma,
+                             Pm.un, Pm.ut, Pm.p, Pm.a);
+        P[0].r = static_cast<TF>(qr[0].x);
+        cons_to_prim2<TF, M>(P[0].r, static_cast<TF>(qm[0].x), static_cast<TF>(qm[0].y), static_cast<TF>(qr[0].y),
+                             gmo, gamma, P[0].un, P[0].ut, P[0].p, P[0].a);
+        face_flux2<TF, M>(Pm, P[0], K, divgamma, F[0][0], F[0][1], F[0][2], F[0][3]);
+    }
+    int i = i0;
+    auto step = [&](auto phase) -> bool {
+        constexpr int p = decltype(phase)::value;
+        constexpr int sc = p % S, sn = (p + 1) % S, sl = (p + kPF + 1) % S;
+        constexpr int pc = p % 2, pn = (p + 1) % 2;
+        // Prefetch row i+1+kPF (clamped to the last row this strip reads) into the slot
+        // row i-1 left.
+        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i + 1 + kPF, i1 + 1), nb_x) * ys + jl, qr[sl], qm[sl]);
+        key = row2d<TS, TF, M>(qr[sc], qm[sc], qr[sn], qm[sn], P[pc], P[pn], F[pc], F[pn], RN2, MN2, ys, i, j, ymask,
+                               owned, gamma, gmo, divgamma, K, DtDx, DtDy, key);
+        return ++i <= i1;
+    };
+    while (run_phases(step, std::make_integer_sequence<int, U>{})) {
+    }
+#else
+    using V2 = typename Vec2<TS>::type;
     V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
     load_cell<TS>(R2, M2, row_of<WRAP_X>(i0 - 1, nb_x) * ys + jl, ra, ma);
     load_cell<TS>(R2, M2, row_of<WRAP_X>(i0, nb_x) * ys + jl, rc, mc);
@@ -150,82 +266,28 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
     Pc.r = static_cast<TF>(rc.x);
     cons_to_prim2<TF, M>(Pc.r, static_cast<TF>(mc.x), static_cast<TF>(mc.y), static_cast<TF>(rc.y), gmo, gamma, Pc.un,
                          Pc.ut, Pc.p, Pc.a);
-    TF fx1_0, fx1_1, fx1_2, fx1_3;
-    face_flux2<TF, M>(Pm, Pc, K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3);
-
+    TF F1[4];
+    face_flux2<TF, M>(Pm, Pc, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
     for (int i = i0; i <= i1; ++i) {
         // Prefetch row i+1+kPF (clamped to the last row this strip reads).
-        const int ipf = min(i + 1 + kPF, i1 + 1);
-        load_cell<TS>(R2, M2, row_of<WRAP_X>(ipf, nb_x) * ys + jl, qr[kPF], qm[kPF]);
-        const V2 rn = qr[0], mn = qm[0];  // row i+1
-
-        // Y face (j, j+1) of row i in the (rho, v, u, p) frame (kernel.cxx:213-251),
-        // evaluated by lane j and handed to lane j+1.
-        Prim2<TF> Sy, Ny;
-        Sy.r = Pc.r;
-        Sy.un = xor_sign(Pc.ut, ymask);
-        Sy.ut = Pc.un;
-        Sy.p = Pc.p;
-        Sy.a = Pc.a;
-        Ny.r = from_next_lane(Sy.r);
-        Ny.un = from_next_lane(Sy.un);
-        Ny.ut = from_next_lane(Sy.ut);
-        Ny.p = from_next_lane(Sy.p);
-        Ny.a = from_next_lane(Sy.a);
-        TF fyR0, fyR1, fyR2, fyR3;
-        face_flux2<TF, M>(Sy, Ny, K, divgamma, fyR0, fyR1, fyR2, fyR3);
-        const TF fyL0 = from_prev_lane(fyR0);
-        const TF fyL1 = from_prev_lane(fyR1);
-        const TF fyL2 = from_prev_lane(fyR2);
-        const TF fyL3 = from_prev_lane(fyR3);
-
-        // Row i+1 primitives and the X face (i, i+1).
+        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i + 1 + kPF, i1 + 1), nb_x) * ys + jl, qr[kPF], qm[kPF]);
         Prim2<TF> Pn;
-        Pn.r = static_cast<TF>(rn.x);
-        cons_to_prim2<TF, M>(Pn.r, static_cast<TF>(mn.x), static_cast<TF>(mn.y), static_cast<TF>(rn.y), gmo, gamma,
-                             Pn.un, Pn.ut, Pn.p, Pn.a);
-        TF fx2_0, fx2_1, fx2_2, fx2_3;
-        face_flux2<TF, M>(Pc, Pn, K, divgamma, fx2_0, fx2_1, fx2_2, fx2_3);
-
-        // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the
-        // (rho, v, u, E) frame, so rho*u takes component 2 and rho*v component 1.
-        // Increments and update are in the state precision TS.
-        const TS x_r = DtDx * (static_cast<TS>(fx1_0) - static_cast<TS>(fx2_0));
-        const TS x_u = DtDx * (static_cast<TS>(fx1_1) - static_cast<TS>(fx2_1));
-        const TS x_v = DtDx * (static_cast<TS>(fx1_2) - static_cast<TS>(fx2_2));
-        const TS x_E = DtDx * (static_cast<TS>(fx1_3) - static_cast<TS>(fx2_3));
-        const TS y_r = DtDy * (static_cast<TS>(fyL0) - static_cast<TS>(fyR0));
-        const TS y_u = DtDy * (static_cast<TS>(fyL2) - static_cast<TS>(fyR2));
-        const TS y_v = DtDy * (static_cast<TS>(fyL1) - static_cast<TS>(fyR1));
-        const TS y_E = DtDy * (static_cast<TS>(fyL3) - static_cast<TS>(fyR3));
-        const TS nr = rc.x + x_r + y_r;
-        const TS nu = mc.x + x_u + y_u;
-        const TS nv = mc.y + x_v + y_v;
-        const TS nE = rc.y + x_E + y_E;
-        const TF sp = signal_speed2<TF, M>(static_cast<TF>(nr), static_cast<TF>(nu), static_cast<TF>(nv),
-                                           static_cast<TF>(nE), gmo, gamma);
-        const uint64_t k_new = speed_key(static_cast<double>(sp));
-        key = owned ? umax64(key, k_new) : key;
-        if (owned) {
-            const long long n = static_cast<long long>(i) * ys + j;
-            store_pair<TS>(RN2 + n, nr, nE);
-            store_pair<TS>(MN2 + n, nu, nv);
-        }
-
+        TF F2[4];
+        key = row2d<TS, TF, M>(rc, mc, qr[0], qm[0], Pc, Pn, F1, F2, RN2, MN2, ys, i, j, ymask, owned, gamma, gmo,
+                               divgamma, K, DtDx, DtDy, key);
         // Advance the march.
         Pc = Pn;
-        rc = rn;
-        mc = mn;
+        rc = qr[0];
+        mc = qm[0];
 #pragma unroll
         for (int q = 0; q < kPF; ++q) {
             qr[q] = qr[q + 1];
             qm[q] = qm[q + 1];
         }
-        fx1_0 = fx2_0;
-        fx1_1 = fx2_1;
-        fx1_2 = fx2_2;
-        fx1_3 = fx2_3;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) F1[q] = F2[q];
     }
+#endif
     return key;
 }
 
