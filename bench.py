@@ -25,6 +25,10 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+# Published compute rate of the same workload (2D 40000 x 20000 FP64 blast) on the
+# reference's hardware: 1018.004 Mc/s on two IA-840F FPGAs
+# (hydro2d/reference_run/res_2000_40000_20000_2FPGAS.res:10108).
+PUBLISHED_GCUPS = {"blast2d_40000x20000_fp64": 1.018004}
 
 WORKLOADS = {
     # name: (dim, nb, C, ic_mode, radius_frac, p_in, p_out, bytes per cell update, precision)
@@ -98,6 +102,50 @@ def cpu_baseline(dim: int, budget_s: float = 14.0, serial_budget_s: float = 8.0)
         "serial": {"value": round(ser, 5), "unit": "GCUPS", "cores": 1,
                    "sample": f"{ser_it} steps, 1 thread ({ser_s:.1f} s)"},
     }
+
+
+class EnergyMeter:
+    """GPU energy over an interval from the amdsmi energy accumulator (the GPU analogue of
+    the reference's energy_record.sh power sampling).  Unavailable counters leave
+    `error` set and readings None."""
+
+    def __init__(self):
+        self.error = None
+        self.h = None
+        try:
+            import amdsmi
+            import torch
+
+            self.smi = amdsmi
+            amdsmi.amdsmi_init()
+            handles = amdsmi.amdsmi_get_processor_handles()
+            want = None
+            try:
+                pr = torch.cuda.get_device_properties(0)
+                want = (int(pr.pci_domain_id), int(pr.pci_bus_id), int(pr.pci_device_id))
+            except Exception:
+                pass
+            for h in handles:
+                bdf = amdsmi.amdsmi_get_gpu_device_bdf(h)  # "dddd:bb:dd.f"
+                dom, bus, rest = bdf.split(":")
+                if want and (int(dom, 16), int(bus, 16), int(rest.split(".")[0], 16)) == want:
+                    self.h = h
+            if self.h is None and len(handles) == 1:
+                self.h = handles[0]
+            if self.h is None:
+                self.error = f"cannot match the device among {len(handles)} amdsmi handles"
+        except Exception as e:  # noqa: BLE001
+            self.error = f"amdsmi unavailable: {e}"
+
+    def joules(self):
+        if self.h is None:
+            return None
+        try:
+            e = self.smi.amdsmi_get_energy_count(self.h)
+            return e["energy_accumulator"] * e["counter_resolution"] * 1e-6
+        except Exception as e:  # noqa: BLE001
+            self.error = f"energy counter: {e}"
+            return None
 
 
 def load_traffic(workload: str):
@@ -425,14 +473,17 @@ def main() -> int:
     log(f"[bench] warmup {W} steps done, t={st['t']:.4e}")
 
     run.set_timing(True)
+    meter = EnergyMeter()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    j0 = meter.joules()
     w0 = time.perf_counter()
     ev0.record()
     run.advance(a, b, K)
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
+    j1 = meter.joules()
     gpu_s = ev0.elapsed_time(ev1) / 1e3
     kernel_ms, launches = run.kernel_time()
     st = run.status()
@@ -493,7 +544,9 @@ def main() -> int:
         "ms_per_step": round(elapsed / K * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": round(gcups / PUBLISHED_GCUPS[name], 2) if name in PUBLISHED_GCUPS else None,
+        "vs_baseline_basis": ("published 1.018 GCUPS, 2x IA-840F FPGA, res_2000_40000_20000_2FPGAS.res:10108"
+                              if name in PUBLISHED_GCUPS else None),
         "dtype": "f64" if precision == 64 else "f64 state, f32 flux",
         "data": "synthetic (built-in blast IC generated on the device)",
         "config": {"workload": name, "grid": list(nb), "cells": cells, "boundaries": "periodic X, reflective Y/Z",
@@ -506,6 +559,9 @@ def main() -> int:
         "achievable_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
         "frac_of_achievable": round(achieved / copy_gbs, 4) if copy_gbs else None,
         "kernel_ms": round(avg_kernel_s * 1e3, 4),
+        "energy": ({"joules_per_step": round((j1 - j0) / K, 4), "nj_per_cell_update": round((j1 - j0) / K / cells * 1e9, 4),
+                    "avg_watts": round((j1 - j0) / wall, 1), "source": "amdsmi energy accumulator over the timed steps"}
+                   if j0 is not None and j1 is not None and j1 > j0 else {"error": meter.error or "no energy delta"}),
         "mass_change_per_cell": abs(mass1 - mass0) / cells,
         "t_final": st["t"],
     }
