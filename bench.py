@@ -308,12 +308,17 @@ def bench_served(args) -> dict:
     log("[bench] plain solves, one at a time ...")
     dur = sv.plain_solve_times(hydro, torch, reqs, progress=prog)
     torch.cuda.empty_cache()
+    log("[bench] the same solves in the reference's loop shape (synchronous launcher per step) ...")
+    dur_sync = sv.plain_solve_times(hydro, torch, reqs, progress=prog, mode="sync")
+    torch.cuda.empty_cache()
     mean_d = sum(dur) / n
     rate = 0.8 / mean_d
     for r in reqs:
         r["arrival"] = r["arrival"] / rate
     seq_lat = sv.sequential_latencies(reqs, dur)
     seq_finish = max(r["arrival"] + l for r, l in zip(reqs, seq_lat))
+    sync_lat = sv.sequential_latencies(reqs, dur_sync)
+    sync_finish = max(r["arrival"] + l for r, l in zip(reqs, sync_lat))
     arena = int(args.serve_hbm_gb * 2**30)
     log(f"[bench] served at {rate:.2f} requests/s (80% of the sequential server) ...")
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
@@ -347,9 +352,15 @@ def bench_served(args) -> dict:
                              "p99_s": round(sv.percentile(seq_lat, 99), 4),
                              "gcups": round(work / (seq_finish - reqs[0]["arrival"]) / 1e9, 3),
                              "mean_solve_s": round(mean_d, 4)},
+        "sequential_sync_launcher": {"p50_s": round(sv.percentile(sync_lat, 50), 4),
+                                     "p99_s": round(sv.percentile(sync_lat, 99), 4),
+                                     "gcups": round(work / (sync_finish - reqs[0]["arrival"]) / 1e9, 3),
+                                     "mean_solve_s": round(sum(dur_sync) / n, 4)},
         "saturation": {"served_gcups": round(sat["gcups"], 3), "sequential_plain_gcups": round(seq_gcups, 3),
-                       "speedup": round(sat["gcups"] / seq_gcups, 3), "served_p50_s": round(sat["p50"], 4),
-                       "served_p99_s": round(sat["p99"], 4)},
+                       "speedup": round(sat["gcups"] / seq_gcups, 3),
+                       "sequential_sync_launcher_gcups": round(work / sum(dur_sync) / 1e9, 3),
+                       "speedup_vs_sync_launcher": round(sat["gcups"] / (work / sum(dur_sync) / 1e9), 3),
+                       "served_p50_s": round(sat["p50"], 4), "served_p99_s": round(sat["p99"], 4)},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -582,7 +593,8 @@ def main() -> int:
             | {"tiling": bt["config"]["tiling"], "roofline_achieved_gbs": bt["roofline"]["achieved"]},
             "served_mixed": {"requests": sv["config"]["requests"], "arrival_rate_per_s": sv["config"]["arrival_rate_per_s"],
                              "load": sv["config"]["load"], "served": sv["served"],
-                             "sequential_plain": sv["sequential_plain"], "saturation": sv["saturation"]},
+                             "sequential_plain": sv["sequential_plain"],
+                             "sequential_sync_launcher": sv["sequential_sync_launcher"], "saturation": sv["saturation"]},
         }
     print(json.dumps(line), flush=True)
     return 0

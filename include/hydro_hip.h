@@ -298,6 +298,13 @@ int hydro_server_set_limits(hydro_server* server, double max_active_cells, size_
 int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision, double* mass,
                      void* stream);
 
+/* Total energy (sum of interior E), the same compensated FP64 reduction.  No counterpart
+ * in the reference, which tracks mass only; it is the second conserved total the
+ * FP32-flux form is checked on (periodic X and reflective walls conserve it exactly in
+ * exact arithmetic). */
+int hydro_total_energy(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision, double* energy,
+                       void* stream);
+
 /* Copy `bytes` (a multiple of 16) from src to dst on the device with a plain streaming
  * kernel: the achievable-bandwidth reference for the step kernels, which read and write
  * the state once.  No counterpart in the reference (measurement only). */

@@ -85,6 +85,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_tune_set.argtypes = [C.c_char_p, i32]
     L.hydro_tune_get.argtypes = [C.c_char_p]
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
+    L.hydro_total_energy.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_stream_copy.argtypes = [vp, vp, sz, vp]
     L.hydro_batch_create.argtypes = [i32, i32, sz]
     L.hydro_batch_create.restype = vp
@@ -254,6 +255,16 @@ def total_mass(rhoE, stream=None) -> float:
     m = C.c_double()
     _check(lib().hydro_total_mass(_p(rhoE), nb[0], nb[1], nb[2] if dim == 3 else 0, 64 if _is64(rhoE) else 32,
                                   C.byref(m), _stream(stream)), "hydro_total_mass")
+    return m.value
+
+
+def total_energy(rhoE, stream=None) -> float:
+    """Sum of interior E (compensated, on the device)."""
+    dim = rhoE.dim() - 1
+    nb = [s - 2 for s in rhoE.shape[:dim]]
+    m = C.c_double()
+    _check(lib().hydro_total_energy(_p(rhoE), nb[0], nb[1], nb[2] if dim == 3 else 0, 64 if _is64(rhoE) else 32,
+                                    C.byref(m), _stream(stream)), "hydro_total_energy")
     return m.value
 
 

@@ -300,11 +300,12 @@ __device__ __forceinline__ void dd_add(double& hi, double& lo, double bhi, doubl
 }
 
 // Interior cell (i, j, k) in [0, nb_x) x [0, nb_y) x [0, nb_z) lives at element
-// base + i*sx + j*sy + k*sz of the cell array (2D: nb_z = 1, sz = 0).
+// base + i*sx + j*sy + k*sz of the cell array (2D: nb_z = 1, sz = 0); comp 0 sums rho,
+// comp 1 sums E.
 template <typename T>
 __global__ void __launch_bounds__(256) mass_kernel(const T* __restrict__ rhoE, long long nb_x, long long nb_y,
                                                    long long nb_z, long long base, long long sx, long long sy,
-                                                   long long sz, double* partial)
+                                                   long long sz, int comp, double* partial)
 {
     const long long ncell = nb_x * nb_y * nb_z;
     double hi = 0.0, lo = 0.0;
@@ -315,7 +316,7 @@ __global__ void __launch_bounds__(256) mass_kernel(const T* __restrict__ rhoE, l
         const long long j = q % nb_y;
         const long long i = q / nb_y;
         const long long n = base + i * sx + j * sy + k * sz;
-        dd_add(hi, lo, static_cast<double>(rhoE[2 * n]), 0.0);
+        dd_add(hi, lo, static_cast<double>(rhoE[2 * n + comp]), 0.0);
     }
     __shared__ double sh[256], sl[256];
     sh[threadIdx.x] = hi;
@@ -346,13 +347,16 @@ __global__ void mass_finish_kernel(const double* partial, int n, double* out)
 
 }  // namespace
 
-extern "C" int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision,
-                                double* mass, void* stream)
+namespace {
+
+// Compensated (double-double) device sum of component `comp` of the interior rhoE pairs.
+int total_component(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision, int comp, double* out,
+                    void* stream, const char* fn)
 {
     clear_error();
     if (precision == HYDRO_FP32_FLUX) precision = HYDRO_FP64;  // FP64 state
-    if (!d_rhoE || !mass || NB_X < 1 || NB_Y < 1 || (precision != 64 && precision != 32)) {
-        set_error("hydro_total_mass: bad arguments");
+    if (!d_rhoE || !out || NB_X < 1 || NB_Y < 1 || (precision != 64 && precision != 32)) {
+        set_error("%s: bad arguments", fn);
         return HYDRO_ERR_ARG;
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
@@ -375,23 +379,37 @@ extern "C" int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, si
     hipError_t e = hipSuccess;
     if (precision == 64)
         hipLaunchKernelGGL(mass_kernel<double>, dim3(nblk), dim3(256), 0, s, static_cast<const double*>(d_rhoE),
-                           static_cast<long long>(NB_X), static_cast<long long>(NB_Y), nz, base, sx, sy, sz,
+                           static_cast<long long>(NB_X), static_cast<long long>(NB_Y), nz, base, sx, sy, sz, comp,
                            d_partial);
     else
         hipLaunchKernelGGL(mass_kernel<float>, dim3(nblk), dim3(256), 0, s, static_cast<const float*>(d_rhoE),
-                           static_cast<long long>(NB_X), static_cast<long long>(NB_Y), nz, base, sx, sy, sz,
+                           static_cast<long long>(NB_X), static_cast<long long>(NB_Y), nz, base, sx, sy, sz, comp,
                            d_partial);
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) {
         hipLaunchKernelGGL(mass_finish_kernel, dim3(1), dim3(64), 0, s, d_partial, nblk, d_partial + 2 * nblk);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(mass, d_partial + 2 * nblk, sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_partial + 2 * nblk, sizeof(double), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d_partial);
     if (e != hipSuccess) {
-        set_error("hydro_total_mass: %s", hipGetErrorString(e));
+        set_error("%s: %s", fn, hipGetErrorString(e));
         return HYDRO_ERR_HIP;
     }
     return 0;
+}
+
+}  // namespace
+
+extern "C" int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision,
+                                double* mass, void* stream)
+{
+    return total_component(d_rhoE, NB_X, NB_Y, NB_Z, precision, 0, mass, stream, "hydro_total_mass");
+}
+
+extern "C" int hydro_total_energy(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision,
+                                  double* energy, void* stream)
+{
+    return total_component(d_rhoE, NB_X, NB_Y, NB_Z, precision, 1, energy, stream, "hydro_total_energy");
 }

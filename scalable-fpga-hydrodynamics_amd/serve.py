@@ -67,9 +67,12 @@ def run_trace(hydro, server, reqs, realtime: bool = True, keep: bool = False, pr
             "gcups": work / (last - first) / 1e9, "p50": percentile(lat, 50), "p99": percentile(lat, 99)}
 
 
-def plain_solve_times(hydro, torch, reqs, progress=None) -> list:
-    """Seconds per request solved alone on the device-resident single-solve loop (IC on
-    the device, then max_it = steps steps), one after another."""
+def plain_solve_times(hydro, torch, reqs, progress=None, mode: str = "resident") -> list:
+    """Seconds per request solved alone, one after another, IC on the device:
+    mode "resident": the device-resident single-solve loop (hydro_run, max_it = steps);
+    mode "sync": the reference's loop shape (hydro2d/main.cxx:712-815) -- per step a
+    ghost fill, a synchronous launcher and the Dt read back to the host, DtDx / DtDy
+    recomputed on the host."""
     out = []
     for j, r in enumerate(reqs):
         nb = r["nb"]
@@ -80,15 +83,34 @@ def plain_solve_times(hydro, torch, reqs, progress=None) -> list:
         k = hydro.constants(nb, r["C"])
         dt0 = hydro.ic(a[0], a[1], k["C_min"], ic_mode=1, radius_frac=r["radius_frac"], p_in=r["p_in"],
                        p_out=r["p_out"], rho_in=r["rho_in"], rho_out=r["rho_out"])
-        run = hydro.Run(nb, k["D"], k["C_min"], dt0, 1e30, r["steps"], _precision(hydro, r))
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        run.advance(a, b, r["steps"])
-        st = run.status()
-        out.append(time.perf_counter() - t0)
-        if st["rc"] != 0 or st["it"] != r["steps"]:
-            raise RuntimeError(f"plain solve of request {j} failed: {st}")
-        run.close()
+        if mode == "sync":
+            dtn = torch.zeros(1, dtype=torch.float64, device="cuda")
+            cur, nxt = a, b
+            dt = dt0
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(r["steps"]):
+                if len(nb) == 2:
+                    hydro.ghost_fill2d(cur[0], cur[1])
+                    hydro.launcher(cur[0], cur[1], nxt[0], nxt[1], dtn, k["C_min"], k["gamma"], k["gmo"],
+                                   k["divgamma"], k["K"], dt / k["D"][0], dt / k["D"][1])
+                else:
+                    hydro.ghost_fill3d(cur[0], cur[1])
+                    hydro.launcher3d(cur[0], cur[1], nxt[0], nxt[1], dtn, k["C_min"], k["gamma"], k["gmo"],
+                                     k["divgamma"], k["K"], dt / k["D"][0], dt / k["D"][1], dt / k["D"][2])
+                dt = float(dtn.item())
+                cur, nxt = nxt, cur
+            out.append(time.perf_counter() - t0)
+        else:
+            run = hydro.Run(nb, k["D"], k["C_min"], dt0, 1e30, r["steps"], _precision(hydro, r))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            run.advance(a, b, r["steps"])
+            st = run.status()
+            out.append(time.perf_counter() - t0)
+            if st["rc"] != 0 or st["it"] != r["steps"]:
+                raise RuntimeError(f"plain solve of request {j} failed: {st}")
+            run.close()
         del a, b
         if progress:
             progress(j + 1, len(reqs))

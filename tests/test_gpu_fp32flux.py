@@ -22,15 +22,20 @@ def _run(gpu, nb, steps, precision, ic_mode=0, radius_frac=0.0):
     b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
     dt0 = gpu.ic(a[0], a[1], k["C_min"], ic_mode=ic_mode, radius_frac=radius_frac)
     m0 = gpu.total_mass(a[0])
+    e0 = gpu.total_energy(a[0])
     run = gpu.Run(nb, k["D"], k["C_min"], dt0, 10.0, steps, precision)
     run.advance(a, b, steps)
     st = run.status()
     assert st["rc"] == 0 and st["it"] == steps, st
     cur = a if st["current"] == 0 else b
     m1 = gpu.total_mass(cur[0])
+    e1 = gpu.total_energy(cur[0])
     out = (cur[0].cpu().numpy(), cur[1].cpu().numpy())
     tt, td = run.trajectory(steps)
     run.close()
+    # Periodic X and reflective walls (zero wall energy flux) conserve total energy to
+    # summation round-off, in both forms.
+    assert abs(e1 - e0) / e0 < 1e-13, (e0, e1)
     return out, td, abs(m1 - m0) / m0
 
 

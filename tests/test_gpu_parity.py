@@ -230,6 +230,13 @@ def test_total_mass_device(gpu, oracle):
     rhoE3, uvw3, _ = oracle.ic3d(40, 30, 20)
     m3 = gpu.total_mass(to_dev(rhoE3))
     assert abs(m3 - oracle.mass3d(rhoE3)) <= 1e-12 * m3
+    # Total energy: the same reduction over the E component, against an exact sum.
+    import math
+
+    e = gpu.total_energy(to_dev(rhoE))
+    assert abs(e - math.fsum(rhoE[1:-1, 1:-1, 1].ravel().tolist())) <= 1e-13 * e
+    e3 = gpu.total_energy(to_dev(rhoE3))
+    assert abs(e3 - math.fsum(rhoE3[1:-1, 1:-1, 1:-1, 1].ravel().tolist())) <= 1e-13 * e3
 
 
 def test_nonfinite_speed_policy(gpu, oracle):
