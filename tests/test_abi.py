@@ -76,3 +76,16 @@ def test_tuning_knobs_round_trip(hydro):
     assert hydro.tune_get("rows2d") == 96
     hydro.tune_set("rows2d", old)
     assert hydro.tune_get("rows3d") >= 2
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without the HIP library the package raises; there is no CPU fallback."""
+    import subprocess
+    import sys
+
+    code = ("import importlib, sys; sys.path.insert(0, %r); "
+            "h = importlib.import_module('scalable-fpga-hydrodynamics_amd'); "
+            "\ntry:\n    h.lib()\nexcept h.HydroError as e:\n    print('raised', e)\n    sys.exit(0)\nsys.exit(1)") % str(ROOT)
+    env = {"HYDRO_LIB_PATH": str(tmp_path / "missing.so"), "PATH": "/usr/bin:/bin"}
+    p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0 and "is not built" in p.stdout, (p.stdout, p.stderr)
