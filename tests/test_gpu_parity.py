@@ -1,7 +1,7 @@
 """GPU parity: the HIP kernels against the CPU oracle, bit for bit.
 
-Every comparison is bitwise on the IEEE representation (so signed zeros count); the
-north-star tolerance is 1e-12 relative L-inf, and the kernels meet it with zero error
+Every comparison is bitwise on the IEEE representation (so signed zeros count).  The
+FP64 bar is a relative L-inf error of at most 1e-12; the kernels meet it with zero error
 because they perform the reference's operations in the reference's order.
 """
 import numpy as np
