@@ -183,7 +183,8 @@ int hydro_ic_f32(float* d_rhoE, float* d_mom, int dim, const size_t* nb, const s
  * advances its t / iteration count and flips its current buffer; a blast stops on its
  * own when !(t < max_t && it < max_it).  Steps may be replayed from a hipGraph.  The
  * batch runs on its own non-blocking stream (hydro_batch_stream); buffers handed to it
- * must be ready before the first advance (synchronise the producing stream).
+ * must be ready before the first advance (synchronise the producing stream).  A batch
+ * is used from one host thread at a time; distinct batches may run concurrently.
  * ------------------------------------------------------------------------------- */
 
 typedef struct hydro_batch hydro_batch;
@@ -249,8 +250,9 @@ int hydro_batch_autotune(hydro_batch* batch, size_t steps_per_trial, int use_cac
  * ghosts, 256-byte aligned -- fits in the server's HBM arena (one allocation made at
  * create) and their engine has a free slot.  One engine per (dimension, precision), each
  * on its own stream; a request joins between ticks and leaves when its own loop ends.
- * hydro_server_tick: admit, advance every engine by steps_per_tick, wait, retire.  Times
- * are seconds on the server's steady clock (hydro_server_now).
+ * hydro_server_tick serves every idle engine (retire, admit, enqueue steps_per_tick
+ * steps) and waits only while every engine with work is busy.  Times are seconds on the
+ * server's steady clock (hydro_server_now).  A server is used from one host thread.
  * ------------------------------------------------------------------------------- */
 
 typedef struct hydro_server hydro_server;

@@ -65,7 +65,10 @@ struct BatchSeg {
 
 namespace {
 
-constexpr int kBatchMinBlocks2 = 4;
+// Waves per SIMD of the 2D batch kernel (= workgroups per CU): the FP32-flux form fits
+// five in 96 VGPRs without spilling, the FP64 form four (up to 128 VGPRs).
+template <typename TF>
+constexpr int kBatchWaves2 = sizeof(TF) == 4 ? 5 : 4;
 
 __device__ __forceinline__ void flush_key(BatchBlast* __restrict__ blasts, int b, uint64_t key, unsigned wid, int lane)
 {
@@ -74,7 +77,7 @@ __device__ __forceinline__ void flush_key(BatchBlast* __restrict__ blasts, int b
 }
 
 template <typename TS, typename TF, typename M>
-__global__ void __launch_bounds__(kBlock, kBatchMinBlocks2)
+__global__ void __launch_bounds__(kBlock, kBatchWaves2<TF>)
     batch2d_kernel(BatchBlast* __restrict__ blasts, const BatchSeg* __restrict__ segs,
                    const unsigned* __restrict__ seg_off, Consts<double> cs, unsigned nworkers, int group)
 {
@@ -258,9 +261,13 @@ struct hydro_batch {
 
 namespace {
 
-int default_blocks(const hydro_batch* b) { return b->dim == 2 ? kBatchMinBlocks2 : 2; }
+int max_blocks(const hydro_batch* b)
+{
+    if (b->dim == 3) return 3;
+    return b->precision == HYDRO_FP32_FLUX ? kBatchWaves2<float> : kBatchWaves2<double>;
+}
 
-int max_blocks(const hydro_batch* b) { return b->dim == 2 ? kBatchMinBlocks2 : 3; }
+int default_blocks(const hydro_batch* b) { return b->dim == 2 ? max_blocks(b) : 2; }
 
 void drop_graph(hydro_batch* b)
 {
