@@ -36,9 +36,13 @@ WORKLOADS = {
     "blast3d_512_fp64": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80, 64),
     "blast2d_40000x20000_fp32flux": (2, (40000, 20000), 0.5, 0, 0.0, 10.0, 0.1, 64, 3264),
     "blast3d_512_fp32flux": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 80, 3264),
+    # The reference's USE_FLOAT build: FP32 state and arithmetic (32 / 40 B per cell).
+    "blast2d_40000x20000_fp32": (2, (40000, 20000), 0.5, 0, 0.0, 10.0, 0.1, 32, 32),
+    "blast3d_512_fp32": (3, (512, 512, 512), 0.4, 1, 0.1, 10.0, 0.1, 40, 32),
 }
 ALIASES = {"2d": "blast2d_40000x20000_fp64", "3d": "blast3d_512_fp64", "2d-fp32flux": "blast2d_40000x20000_fp32flux",
-           "3d-fp32flux": "blast3d_512_fp32flux", "batched": "batched2d_256_fp32flux",
+           "3d-fp32flux": "blast3d_512_fp32flux", "2d-fp32": "blast2d_40000x20000_fp32", "3d-fp32": "blast3d_512_fp32",
+           "batched": "batched2d_256_fp32flux",
            "batched-fp64": "batched2d_256_fp64"}
 # Batched workloads: 256 seeded 2D blasts (sides log-uniform in [64, 2048]), precision.
 BATCHED = {"batched2d_256_fp32flux": 3264, "batched2d_256_fp64": 64}
@@ -461,12 +465,14 @@ def main() -> int:
     if args.rows:
         hydro.tune_set("rows2d" if dim == 2 else "rows3d", args.rows)
 
-    k = hydro.constants(nb, C_cfl)
+    fp32_state = precision == 32
+    tdt = torch.float32 if fp32_state else torch.float64
+    k = hydro.constants(nb, C_cfl, "f32" if fp32_state else "f64")
     shape = tuple(n + 2 for n in nb)
     mom_c = 2 if dim == 2 else 3
-    log(f"[bench] {name}: allocating {4 * np.prod(shape) * (2 + mom_c) / 2 * 8 / 1e9:.1f} GB")
-    a = [torch.empty(shape + (2,), dtype=torch.float64, device="cuda"),
-         torch.empty(shape + (mom_c,), dtype=torch.float64, device="cuda")]
+    log(f"[bench] {name}: allocating {4 * np.prod(shape) * (2 + mom_c) / 2 * (4 if fp32_state else 8) / 1e9:.1f} GB")
+    a = [torch.empty(shape + (2,), dtype=tdt, device="cuda"),
+         torch.empty(shape + (mom_c,), dtype=tdt, device="cuda")]
     b = [torch.empty_like(a[0]), torch.empty_like(a[1])]
     for t in a + b:
         t.zero_()
@@ -558,7 +564,7 @@ def main() -> int:
         "vs_baseline": round(gcups / PUBLISHED_GCUPS[name], 2) if name in PUBLISHED_GCUPS else None,
         "vs_baseline_basis": ("published 1.018 GCUPS, 2x IA-840F FPGA, res_2000_40000_20000_2FPGAS.res:10108"
                               if name in PUBLISHED_GCUPS else None),
-        "dtype": "f64" if precision == 64 else "f64 state, f32 flux",
+        "dtype": {64: "f64", 32: "f32"}.get(precision, "f64 state, f32 flux"),
         "data": "synthetic (built-in blast IC generated on the device)",
         "config": {"workload": name, "grid": list(nb), "cells": cells, "boundaries": "periodic X, reflective Y/Z",
                    "steps_timed": K, "kernel": "fvm2d" if dim == 2 else "fvm3d",
