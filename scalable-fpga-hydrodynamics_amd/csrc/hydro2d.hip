@@ -43,6 +43,8 @@ struct Geom2 {
     int rows;      // rows per strip
     int nwin;      // column windows
     int nwb;       // workgroups across Y (nwin / kWavesPerBlock, rounded up)
+    int wrap_x;    // periodic X read from the opposite interior plane (no X ghost fill)
+    int wrap_y;    // periodic Y read from the opposite interior column (no Y ghost fill)
 };
 
 // TS: state and update precision; TF, M: precision and math policy of the primitives,
@@ -80,14 +82,14 @@ __global__ void __launch_bounds__(kBlock, HYDRO_2D_MIN_BLOCKS)
     const bool owned = (win < g.nwin) && lane >= 1 && lane <= kOwned && j <= g.nb_y;
     int jl;
     unsigned int ymask;
-    window_lane2d<TF, WALL_Y, false>(j, g.nb_y, jl, ymask);
+    window_lane2d<TF, WALL_Y>(j, g.nb_y, g.wrap_y != 0, jl, ymask);
     const int i0 = 1 + strip * g.rows;
     const int i1 = min(i0 + g.rows - 1, g.nb_x);
 
     uint64_t key = 0;
     if (win < g.nwin)
-        key = march2d<TS, TF, M, false>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned, gamma, gmo, divgamma,
-                                        K, DtDx, DtDy, key);
+        key = march2d<TS, TF, M>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned, g.wrap_x != 0, gamma, gmo,
+                                 divgamma, K, DtDx, DtDy, key);
 
     block_max_to_slot(red, key);
 }
@@ -148,7 +150,7 @@ constexpr long long kFillBlocks2 = 2048;
 template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
                   T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,
-                  hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr)
+                  hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr, int wraps = 0)
 {
     if (!rhoE || !uv || !rhoE_next || !uv_next || (!ctl && !Dt_next)) {
         set_error("hydro 2D step: null pointer argument");
@@ -168,6 +170,8 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     g.ys = static_cast<long long>(nb_y) + 2;
     g.nwin = static_cast<int>((nb_y + kOwned - 1) / kOwned);
     g.nwb = (g.nwin + kWavesPerBlock - 1) / kWavesPerBlock;
+    g.wrap_x = (wraps & kWrapX) ? 1 : 0;
+    g.wrap_y = (wraps & kWrapY) ? 1 : 0;
     g.rows = g_rows_2d;
     while (g.rows > 8 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nwb < kFillBlocks2) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
@@ -291,28 +295,28 @@ double timed_launcher2d(const T* d_rhoE, const T* d_uv, T* d_rhoE_next, T* d_uv_
 
 int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
-                    void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
+                    void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b, int wraps)
 {
     if (precision == HYDRO_FP32_FLUX) {
         const Consts<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step2d<double, float, FastMath>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
                                                       static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c,
                                                       nb_x, nb_y, 0.0, 0.0, bc_y, red, ctl, nullptr, traj_t,
-                                                      static_cast<double*>(traj_dt), s, ev_a, ev_b);
+                                                      static_cast<double*>(traj_dt), s, ev_a, ev_b, wraps);
     }
     if (precision == 64) {
         const Consts<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step2d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
                                      static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c, nb_x, nb_y,
                                      0.0, 0.0, bc_y, red, ctl, nullptr, traj_t, static_cast<double*>(traj_dt), s, ev_a,
-                                     ev_b);
+                                     ev_b, wraps);
     }
     const Consts<float> c{ static_cast<float>(consts[0]), static_cast<float>(consts[1]),
                            static_cast<float>(consts[2]), static_cast<float>(consts[3]),
                            static_cast<float>(consts[4]) };
     return launch_step2d<float>(static_cast<const float*>(rhoE), static_cast<const float*>(mom),
                                 static_cast<float*>(rhoE_next), static_cast<float*>(mom_next), c, nb_x, nb_y, 0.f, 0.f,
-                                bc_y, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s, ev_a, ev_b);
+                                bc_y, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s, ev_a, ev_b, wraps);
 }
 
 int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,

@@ -37,6 +37,7 @@ struct Geom3 {
     long long zs, zy;  // nb_z + 2, (nb_y + 2) * (nb_z + 2)
     int rows;          // X rows per strip
     int nzw, nyw;      // Z windows, Y tiles
+    int wrap_x, wrap_y, wrap_z;  // periodic axes read from the opposite interior plane
 };
 
 // TS: state and update precision; TF, M: precision and math policy of the primitives,
@@ -87,20 +88,22 @@ __global__ void __launch_bounds__(kBlock3, HYDRO_3D_MIN_WAVES)
         ymask = (j == 0 || j == g.nb_y + 1) ? kSignMask<TF> : 0u;
     } else {
         jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
+        if (g.wrap_y) jl = j == 0 ? g.nb_y : (jl == g.nb_y + 1 ? 1 : jl);
     }
     if constexpr (WALL_Z) {
         kl = k < 1 ? 1 : (k > g.nb_z ? g.nb_z : k);
         zmask = (k == 0 || k == g.nb_z + 1) ? kSignMask<TF> : 0u;
     } else {
         kl = k <= g.nb_z + 1 ? k : g.nb_z + 1;
+        if (g.wrap_z) kl = k == 0 ? g.nb_z : (kl == g.nb_z + 1 ? 1 : kl);
     }
     const long long col = static_cast<long long>(jl) * g.zs + kl;
     const int i0 = 1 + strip * g.rows;
     const int i1 = min(i0 + g.rows - 1, g.nb_x);
 
     __shared__ March3Lds<TF> lds;
-    const uint64_t key = march3d<TS, TF, M, false>(R2, M3, RN2, MN3, g.zy, col, g.nb_x, i0, i1, w, lane, halo_wave, owned,
-                                                   ymask, zmask, gamma, gmo, divgamma, K, DtDx, DtDy, DtDz, lds, 0);
+    const uint64_t key = march3d<TS, TF, M>(R2, M3, RN2, MN3, g.zy, col, g.nb_x, i0, i1, w, lane, halo_wave, owned, ymask,
+                                            zmask, g.wrap_x != 0, gamma, gmo, divgamma, K, DtDx, DtDy, DtDz, lds, 0);
     block_max_to_slot(red, key);
 }
 
@@ -180,7 +183,7 @@ template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
                   size_t nb_y, size_t nb_z, T DtDx, T DtDy, T DtDz, int bc_y, int bc_z, StepReduce* red,
                   StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
-                  hipEvent_t ev_b = nullptr)
+                  hipEvent_t ev_b = nullptr, int wraps = 0)
 {
     if (!rhoE || !uvw || !rhoE_next || !uvw_next || (!ctl && !Dt_next)) {
         set_error("hydro 3D step: null pointer argument");
@@ -203,6 +206,9 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     g.zy = (static_cast<long long>(nb_y) + 2) * g.zs;
     g.nzw = static_cast<int>((nb_z + kOwnedZ - 1) / kOwnedZ);
     g.nyw = static_cast<int>((nb_y + kTY - 1) / kTY);
+    g.wrap_x = (wraps & kWrapX) ? 1 : 0;
+    g.wrap_y = (wraps & kWrapY) ? 1 : 0;
+    g.wrap_z = (wraps & kWrapZ) ? 1 : 0;
     g.rows = g_rows_3d;
     while (g.rows > 4 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nzw * g.nyw < kFillBlocks3) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
@@ -332,21 +338,22 @@ double timed_launcher3d(const T* d_rhoE, const T* d_uvw, T* d_rhoE_next, T* d_uv
 
 int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
-                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b)
+                    StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s, hipEvent_t ev_a, hipEvent_t ev_b,
+                    int wraps)
 {
     if (precision == HYDRO_FP32_FLUX) {
         const Consts3<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step3d<double, float, FastMath>(
             static_cast<const double*>(rhoE), static_cast<const double*>(mom), static_cast<double*>(rhoE_next),
             static_cast<double*>(mom_next), c, nb_x, nb_y, nb_z, 0.0, 0.0, 0.0, bc_y, bc_z, red, ctl, nullptr, traj_t,
-            static_cast<double*>(traj_dt), s, ev_a, ev_b);
+            static_cast<double*>(traj_dt), s, ev_a, ev_b, wraps);
     }
     if (precision == 64) {
         const Consts3<double> c{ consts[0], consts[1], consts[2], consts[3], consts[4] };
         return launch_step3d<double>(static_cast<const double*>(rhoE), static_cast<const double*>(mom),
                                      static_cast<double*>(rhoE_next), static_cast<double*>(mom_next), c, nb_x, nb_y,
                                      nb_z, 0.0, 0.0, 0.0, bc_y, bc_z, red, ctl, nullptr, traj_t,
-                                     static_cast<double*>(traj_dt), s, ev_a, ev_b);
+                                     static_cast<double*>(traj_dt), s, ev_a, ev_b, wraps);
     }
     const Consts3<float> c{ static_cast<float>(consts[0]), static_cast<float>(consts[1]),
                             static_cast<float>(consts[2]), static_cast<float>(consts[3]),
@@ -354,7 +361,7 @@ int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mo
     return launch_step3d<float>(static_cast<const float*>(rhoE), static_cast<const float*>(mom),
                                 static_cast<float*>(rhoE_next), static_cast<float*>(mom_next), c, nb_x, nb_y, nb_z,
                                 0.f, 0.f, 0.f, bc_y, bc_z, red, ctl, nullptr, traj_t, static_cast<float*>(traj_dt), s, ev_a,
-                                ev_b);
+                                ev_b, wraps);
 }
 
 int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,

@@ -122,9 +122,9 @@ __global__ void __launch_bounds__(kBlock, kBatchWaves2<TF>)
         const bool owned = lane >= 1 && lane <= kOwned && j <= nb_y;
         int jl;
         unsigned int ymask;
-        window_lane2d<TF, true, false>(j, nb_y, jl, ymask);
-        key = march2d<TS, TF, M, true>(R2, M2, RN2, MN2, ys, nb_x, sg.i0, sg.i1, j, jl, ymask, owned, gamma, gmo,
-                                       divgamma, K, DtDx, DtDy, key);
+        window_lane2d<TF, true>(j, nb_y, false, jl, ymask);
+        key = march2d<TS, TF, M>(R2, M2, RN2, MN2, ys, nb_x, sg.i0, sg.i1, j, jl, ymask, owned, true, gamma, gmo,
+                                 divgamma, K, DtDx, DtDy, key);
     }
     if (cb >= 0) flush_key(blasts, cb, key, wid, lane);
 }
@@ -172,10 +172,10 @@ __global__ void __launch_bounds__(kBlock3, 4)
         const int kl = k < 1 ? 1 : (k > nb_z ? nb_z : k);
         const unsigned int zmask = (k == 0 || k == nb_z + 1) ? kSignMask<TF> : 0u;
         const long long col = static_cast<long long>(jl) * B->zs + kl;
-        key = march3d<TS, TF, M, true>(static_cast<const V2*>(B->rhoE[cur]), static_cast<const Trip<TS>*>(B->mom[cur]),
-                                       static_cast<V2*>(B->rhoE[cur ^ 1u]), static_cast<Trip<TS>*>(B->mom[cur ^ 1u]),
-                                       B->zy, col, nb_x, sg.i0, sg.i1, w, lane, halo_wave, owned, ymask, zmask, gamma,
-                                       gmo, divgamma, K, DtDx, DtDy, DtDz, lds, key);
+        key = march3d<TS, TF, M>(static_cast<const V2*>(B->rhoE[cur]), static_cast<const Trip<TS>*>(B->mom[cur]),
+                                 static_cast<V2*>(B->rhoE[cur ^ 1u]), static_cast<Trip<TS>*>(B->mom[cur ^ 1u]), B->zy,
+                                 col, nb_x, sg.i0, sg.i1, w, lane, halo_wave, owned, ymask, zmask, true, gamma, gmo,
+                                 divgamma, K, DtDx, DtDy, DtDz, lds, key);
     }
     if (cb >= 0) flush_key(blasts, cb, key, wid, lane);
 }

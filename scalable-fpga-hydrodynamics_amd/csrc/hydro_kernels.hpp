@@ -8,15 +8,20 @@
 
 namespace hydro {
 
+// Periodic boundaries read from the opposite interior plane / line inside the step
+// kernel, so that no ghost fill is needed for them (bits of `wraps`).
+constexpr int kWrapX = 1, kWrapY = 2, kWrapZ = 4;
+
 // consts = { C_min, gamma, gamma_minus_one, divgamma, K } in the state precision.  When
 // ev_a / ev_b are given they are recorded around the step kernel (not the finalise).
 int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     int bc_y, int precision, const double* consts, StepControl* ctl, StepReduce* red, double* traj_t,
-                    void* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr);
+                    void* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr,
+                    int wraps = 0);
 int step3d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,
                     size_t nb_z, int bc_y, int bc_z, int precision, const double* consts, StepControl* ctl,
                     StepReduce* red, double* traj_t, void* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
-                    hipEvent_t ev_b = nullptr);
+                    hipEvent_t ev_b = nullptr, int wraps = 0);
 
 int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int bc_y, int precision,
                 const StepControl* ctl, hipStream_t s);

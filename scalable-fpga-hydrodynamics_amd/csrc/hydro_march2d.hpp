@@ -95,14 +95,13 @@ __device__ __forceinline__ void load_cell(const typename Vec2<T>::type* __restri
     b = mo[n];
 }
 
-// Row actually read for row r.  WRAP_X: rows 0 and nb_x + 1 are read from rows nb_x and
+// Row actually read for row r.  wrap_x: rows 0 and nb_x + 1 are read from rows nb_x and
 // 1 (the periodic X ghost planes, hydro2d/main.cxx:349-378, without materialising them);
-// otherwise the ghost planes are read as filled by the ghost fill.
-template <bool WRAP_X>
-__device__ __forceinline__ long long row_of(int r, int nb_x)
+// otherwise the ghost planes are read as filled by the ghost fill.  r and wrap_x are
+// wave-uniform, so this is scalar arithmetic.
+__device__ __forceinline__ long long row_of(int r, int nb_x, bool wrap_x)
 {
-    if constexpr (WRAP_X) return r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r);
-    else return r;
+    return wrap_x ? (r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r)) : r;
 }
 
 // Per-lane column set-up for a window whose lane 0 is column j = j_first (Y ghost
@@ -111,19 +110,18 @@ __device__ __forceinline__ long long row_of(int r, int nb_x)
 // wall state (hydro2d/kernel.cxx:214-229, 234-250): lane j = 0 evaluates
 // flux(mirror(c1), c1) for column 1 and lane j = nb_y + 1 hands mirror(c_nb_y) to column
 // nb_y.  Negation is exact, so prims(mirror) == mirror(prims).  Periodic Y reads the
-// ghost columns (filled by the ghost fill, or with WRAP_Y the opposite interior column).
+// ghost columns filled by the ghost fill, or with wrap_y the opposite interior column.
 // Lanes past nb_y + 1 are clamped in bounds and never owned.
-template <typename TF, bool WALL_Y, bool WRAP_Y>
-__device__ __forceinline__ void window_lane2d(int j, int nb_y, int& jl, unsigned int& ymask)
+template <typename TF, bool WALL_Y>
+__device__ __forceinline__ void window_lane2d(int j, int nb_y, bool wrap_y, int& jl, unsigned int& ymask)
 {
     ymask = 0u;
     if constexpr (WALL_Y) {
         jl = j < 1 ? 1 : (j > nb_y ? nb_y : j);
         ymask = (j == 0 || j == nb_y + 1) ? kSignMask<TF> : 0u;
-    } else if constexpr (WRAP_Y) {
-        jl = j == 0 ? nb_y : (j >= nb_y + 1 ? 1 : j);
     } else {
         jl = j <= nb_y + 1 ? j : nb_y + 1;
+        if (wrap_y) jl = j == 0 ? nb_y : (jl == nb_y + 1 ? 1 : jl);
     }
 }
 
@@ -199,7 +197,7 @@ __device__ __forceinline__ bool run_phases(F& step, std::integer_sequence<int, P
 }
 
 // March rows i0..i1 (i0 <= i1) of one window.  Lane columns: j (global, for stores and
-// ownership) and jl (the column loaded).  Returns `key` folded with the order keys of
+// ownership) and jl (the column loaded); wrap_x as in row_of.  Returns `key` folded with the order keys of
 // the new signal speeds of the owned cells.
 //
 // kPF rows of loads are in flight while a row is computed.  HYDRO_2D_UNROLL: rows
@@ -208,23 +206,23 @@ __device__ __forceinline__ bool run_phases(F& step, std::integer_sequence<int, P
 // both rings (lcm(kPF+2, 2) rows), so that slot indices are compile-time constants and
 // advancing the march moves no registers (at the price of register pressure); otherwise
 // the rolled loop shifts the ring by register moves.
-template <typename TS, typename TF, typename M, bool WRAP_X>
+template <typename TS, typename TF, typename M>
 __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __restrict__ R2,
                                             const typename Vec2<TS>::type* __restrict__ M2,
                                             typename Vec2<TS>::type* __restrict__ RN2,
                                             typename Vec2<TS>::type* __restrict__ MN2, long long ys, int nb_x, int i0,
-                                            int i1, int j, int jl, unsigned int ymask, bool owned, TF gamma, TF gmo,
-                                            TF divgamma, TF K, TS DtDx, TS DtDy, uint64_t key)
+                                            int i1, int j, int jl, unsigned int ymask, bool owned, bool wrap_x, TF gamma,
+                                            TF gmo, TF divgamma, TF K, TS DtDx, TS DtDy, uint64_t key)
 {
 #if HYDRO_2D_UNROLL
     using V2 = typename Vec2<TS>::type;
     constexpr int S = kPF + 2;                  // row slots: row r in slot (r - i0) % S
     constexpr int U = (S % 2 == 0) ? S : 2 * S;  // phases of the row ring and the 2-slot rings
     V2 ra, ma, qr[S], qm[S];
-    load_cell<TS>(R2, M2, row_of<WRAP_X>(i0 - 1, nb_x) * ys + jl, ra, ma);
+    load_cell<TS>(R2, M2, row_of(i0 - 1, nb_x, wrap_x) * ys + jl, ra, ma);
 #pragma unroll
     for (int q = 0; q <= kPF; ++q)
-        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i0 + q, i1 + 1), nb_x) * ys + jl, qr[q], qm[q]);
+        load_cell<TS>(R2, M2, row_of(min(i0 + q, i1 + 1), nb_x, wrap_x) * ys + jl, qr[q], qm[q]);
     Prim2<TF> P[2];  // X frame: un = u, ut = v; P[(i - i0) % 2] is row i
     TF F[2][4];      // F[(i - i0) % 2] is the X face (i-1, i)
     {
@@ -244,7 +242,7 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
         constexpr int pc = p % 2, pn = (p + 1) % 2;
         // Prefetch row i+1+kPF (clamped to the last row this strip reads) into the slot
         // row i-1 left.
-        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i + 1 + kPF, i1 + 1), nb_x) * ys + jl, qr[sl], qm[sl]);
+        load_cell<TS>(R2, M2, row_of(min(i + 1 + kPF, i1 + 1), nb_x, wrap_x) * ys + jl, qr[sl], qm[sl]);
         key = row2d<TS, TF, M>(qr[sc], qm[sc], qr[sn], qm[sn], P[pc], P[pn], F[pc], F[pn], RN2, MN2, ys, i, j, ymask,
                                owned, gamma, gmo, divgamma, K, DtDx, DtDy, key);
         return ++i <= i1;
@@ -254,11 +252,11 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
 #else
     using V2 = typename Vec2<TS>::type;
     V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
-    load_cell<TS>(R2, M2, row_of<WRAP_X>(i0 - 1, nb_x) * ys + jl, ra, ma);
-    load_cell<TS>(R2, M2, row_of<WRAP_X>(i0, nb_x) * ys + jl, rc, mc);
+    load_cell<TS>(R2, M2, row_of(i0 - 1, nb_x, wrap_x) * ys + jl, ra, ma);
+    load_cell<TS>(R2, M2, row_of(i0, nb_x, wrap_x) * ys + jl, rc, mc);
 #pragma unroll
     for (int q = 0; q < kPF; ++q)
-        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i0 + 1 + q, i1 + 1), nb_x) * ys + jl, qr[q], qm[q]);
+        load_cell<TS>(R2, M2, row_of(min(i0 + 1 + q, i1 + 1), nb_x, wrap_x) * ys + jl, qr[q], qm[q]);
     Prim2<TF> Pm, Pc;  // X frame: un = u, ut = v
     Pm.r = static_cast<TF>(ra.x);
     cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma, Pm.un,
@@ -270,7 +268,7 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
     face_flux2<TF, M>(Pm, Pc, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
     for (int i = i0; i <= i1; ++i) {
         // Prefetch row i+1+kPF (clamped to the last row this strip reads).
-        load_cell<TS>(R2, M2, row_of<WRAP_X>(min(i + 1 + kPF, i1 + 1), nb_x) * ys + jl, qr[kPF], qm[kPF]);
+        load_cell<TS>(R2, M2, row_of(min(i + 1 + kPF, i1 + 1), nb_x, wrap_x) * ys + jl, qr[kPF], qm[kPF]);
         Prim2<TF> Pn;
         TF F2[4];
         key = row2d<TS, TF, M>(rc, mc, qr[0], qm[0], Pc, Pn, F1, F2, RN2, MN2, ys, i, j, ymask, owned, gamma, gmo,

@@ -109,13 +109,11 @@ __device__ __forceinline__ Prim3<T> frame_z(const P6<T>& q)
     return Prim3<T>{ q.r, q.w, q.v, q.u, q.p, q.a };  // hydro3d/kernel.cxx:296-298
 }
 
-// Row actually read for row r (see row_of in the 2D march): WRAP_X reads the periodic X
+// Row actually read for row r (see row_of in the 2D march): wrap_x reads the periodic X
 // ghost planes from the opposite interior planes.
-template <bool WRAP_X>
-__device__ __forceinline__ long long row3_of(int r, int nb_x)
+__device__ __forceinline__ long long row3_of(int r, int nb_x, bool wrap_x)
 {
-    if constexpr (WRAP_X) return r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r);
-    else return r;
+    return wrap_x ? (r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r)) : r;
 }
 
 template <typename TF>
@@ -126,14 +124,14 @@ struct March3Lds {
 
 // March rows i0..i1 of one (Y x Z) tile; see the file comment.  `col` is the in-plane
 // offset jl * zs + kl of the column this lane loads; stores go to the same column.
-template <typename TS, typename TF, typename M, bool WRAP_X>
+template <typename TS, typename TF, typename M>
 __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __restrict__ R2,
                                             const Trip<TS>* __restrict__ M3,
                                             typename Vec2_3<TS>::type* __restrict__ RN2,
                                             Trip<TS>* __restrict__ MN3, long long zy, long long col, int nb_x, int i0,
                                             int i1, int w, int lane, bool halo_wave, bool owned, unsigned int ymask,
-                                            unsigned int zmask, TF gamma, TF gmo, TF divgamma, TF K, TS DtDx, TS DtDy,
-                                            TS DtDz, March3Lds<TF>& lds, uint64_t key)
+                                            unsigned int zmask, bool wrap_x, TF gamma, TF gmo, TF divgamma, TF K,
+                                            TS DtDx, TS DtDy, TS DtDz, March3Lds<TF>& lds, uint64_t key)
 {
     using V2 = typename Vec2_3<TS>::type;
     auto& s_prim = lds.prim;
@@ -144,23 +142,23 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
     Trip<TS> mc, qm[kPF3 + 1];
     P6<TF> Pc;
     TF fx1_0 = TF(0), fx1_1 = TF(0), fx1_2 = TF(0), fx1_3 = TF(0), fx1_4 = TF(0);
-    load3<TS>(R2, M3, row3_of<WRAP_X>(i0, nb_x) * zy + col, rc, mc);
+    load3<TS>(R2, M3, row3_of(i0, nb_x, wrap_x) * zy + col, rc, mc);
 #pragma unroll
     for (int q = 0; q < kPF3; ++q)
-        load3<TS>(R2, M3, row3_of<WRAP_X>(min(i0 + 1 + q, i1 + 1), nb_x) * zy + col, qr[q], qm[q]);
+        load3<TS>(R2, M3, row3_of(min(i0 + 1 + q, i1 + 1), nb_x, wrap_x) * zy + col, qr[q], qm[q]);
     to_prim<TS, TF, M>(rc, mc, gmo, gamma, Pc);
     if (!halo_wave) {
         V2 ra;
         Trip<TS> ma;
         P6<TF> Pm;
-        load3<TS>(R2, M3, row3_of<WRAP_X>(i0 - 1, nb_x) * zy + col, ra, ma);
+        load3<TS>(R2, M3, row3_of(i0 - 1, nb_x, wrap_x) * zy + col, ra, ma);
         to_prim<TS, TF, M>(ra, ma, gmo, gamma, Pm);
         face_flux3<TF, M>(frame_x(Pm), frame_x(Pc), K, divgamma, fx1_0, fx1_1, fx1_2, fx1_3, fx1_4);
     }
 
     for (int i = i0; i <= i1; ++i) {
         const int ipf = min(i + 1 + kPF3, i1 + 1);
-        load3<TS>(R2, M3, row3_of<WRAP_X>(ipf, nb_x) * zy + col, qr[kPF3], qm[kPF3]);
+        load3<TS>(R2, M3, row3_of(ipf, nb_x, wrap_x) * zy + col, qr[kPF3], qm[kPF3]);
         const V2 rn = qr[0];
         const Trip<TS> mn = qm[0];
 

@@ -148,17 +148,26 @@ extern "C" int hydro_run_advance(hydro_run* r, void* rhoE_a, void* mom_a, void* 
         }
         hipEvent_t ev_a = r->timing ? r->ev[2 * r->ev_used] : nullptr;
         hipEvent_t ev_b = r->timing ? r->ev[2 * r->ev_used + 1] : nullptr;
+        // Periodic axes are read from the opposite interior plane inside the step kernel;
+        // only a reflective X boundary needs its mirrored ghost planes filled first (the
+        // Y and Z walls are applied inside the kernel).
+        const bool px = r->bc[0] == HYDRO_BC_PERIODIC;
+        const int wraps = (px ? kWrapX : 0) | (r->bc[1] == HYDRO_BC_PERIODIC ? kWrapY : 0) |
+                          (r->dim == 3 && r->bc[2] == HYDRO_BC_PERIODIC ? kWrapZ : 0);
+        rc = 0;
         if (r->dim == 2) {
-            rc = ghost2d_any(cr, cm, r->nb[0], r->nb[1], r->bc[0], r->bc[1], r->precision, r->d_ctl, s);
+            if (!px) rc = ghost2d_any(cr, cm, r->nb[0], r->nb[1], r->bc[0], HYDRO_BC_REFLECTIVE, r->precision, r->d_ctl, s);
             if (!rc)
                 rc = step2d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->bc[1], r->precision, r->consts, r->d_ctl,
-                                     &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b);
+                                     &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b, wraps);
         } else {
-            rc = ghost3d_any(cr, cm, r->nb[0], r->nb[1], r->nb[2], r->bc[0], r->bc[1], r->bc[2], r->precision,
-                             r->d_ctl, s);
+            if (!px)
+                rc = ghost3d_any(cr, cm, r->nb[0], r->nb[1], r->nb[2], r->bc[0], HYDRO_BC_REFLECTIVE,
+                                 HYDRO_BC_REFLECTIVE, r->precision, r->d_ctl, s);
             if (!rc)
                 rc = step3d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->nb[2], r->bc[1], r->bc[2], r->precision,
-                                     r->consts, r->d_ctl, &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b);
+                                     r->consts, r->d_ctl, &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b,
+                                     wraps);
         }
         if (!rc && ev_b) ++r->ev_used;
         if (rc) return rc;
