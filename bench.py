@@ -326,10 +326,12 @@ def bench_served(args) -> dict:
     arena = int(args.serve_hbm_gb * 2**30)
     log(f"[bench] served at {rate:.2f} requests/s (80% of the sequential server) ...")
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
+    srv.set_limits(args.serve_cap_cells, args.steps_per_tick)
     live = sv.run_trace(hydro, srv, reqs, realtime=True, progress=prog)
     srv.close()
     log("[bench] served at saturation (all requests at t = 0) ...")
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
+    srv.set_limits(args.serve_cap_cells, args.steps_per_tick)
     sat = sv.run_trace(hydro, srv, reqs, realtime=False, progress=prog)
     srv.close()
     seq_gcups = work / sum(dur) / 1e9
@@ -348,7 +350,8 @@ def bench_served(args) -> dict:
         "data": f"synthetic Poisson stream, splitmix64 seed 0x5EED, {n} requests",
         "config": {"workload": "served_mixed", "requests": n, "mix": "70% 2D 128..4096, 30% 3D 64..256",
                    "steps": "100..2000", "precision": "50% fp32-flux / 50% fp64", "hbm_arena_gb": args.serve_hbm_gb,
-                   "steps_per_tick": args.steps_per_tick, "arrival_rate_per_s": round(rate, 3),
+                   "steps_per_tick": args.steps_per_tick, "engine_cap_cells": args.serve_cap_cells,
+                   "arrival_rate_per_s": round(rate, 3),
                    "load": "0.8 of the one-at-a-time plain server"},
         "served": {"p50_s": round(live["p50"], 4), "p99_s": round(live["p99"], 4), "gcups": round(live["gcups"], 3),
                    "makespan_s": round(live["makespan"], 3)},
@@ -419,6 +422,8 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=120, help="served workload: number of requests")
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0, help="served workload: state arena (GiB)")
     ap.add_argument("--steps-per-tick", type=int, default=20)
+    ap.add_argument("--serve-cap-cells", type=float, default=6e6,
+                    help="served workload: per-engine cap on the interior cells of running requests (0 = none)")
     ap.add_argument("--csv-grid", type=int, nargs=2, default=[6000, 5000], help="csv workload: grid of the driver run")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (the reference for HBM byte counters)")

@@ -285,9 +285,10 @@ int hydro_server_result(hydro_server* server, int id, void* dst_rhoE, void* dst_
 int hydro_server_release(hydro_server* server, int id);
 int hydro_server_stats(hydro_server* server, size_t* ticks, double* cell_updates);
 
-/* Admission cap on the interior cells of running requests (0 = memory and slots only;
- * a request is always admitted when nothing runs) and the steps per tick.  A cap keeps
- * the largest requests from being slowed by sharing the device with many others. */
+/* Admission cap on the interior cells of the running requests of each engine (0 =
+ * memory and slots only; a request is always admitted into an idle engine) and the
+ * steps per tick.  A cap keeps large requests of one kind closer to first-come
+ * first-served instead of sharing their engine with every later arrival. */
 int hydro_server_set_limits(hydro_server* server, double max_active_cells, size_t steps_per_tick);
 
 /* ---------------------------------------------------------------------------------

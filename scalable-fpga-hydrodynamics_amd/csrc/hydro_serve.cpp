@@ -130,8 +130,8 @@ struct hydro_server {
     std::chrono::steady_clock::time_point t0;
     size_t ticks = 0;
     double cell_updates = 0;  // of retired requests
-    double max_active_cells = 0;  // admission cap on interior cells in flight (0 = none)
-    double active_cells = 0;
+    double max_active_cells = 0;  // admission cap on interior cells in flight per engine (0 = none)
+    double active_cells[4] = { 0, 0, 0, 0 };
 
     double now() const
     {
@@ -192,8 +192,11 @@ int admit(hydro_server* s)
             ++q;
             continue;
         }
-        const size_t n_running = s->running[0] + s->running[1] + s->running[2] + s->running[3];
-        if (s->max_active_cells > 0 && n_running > 0 && s->active_cells + r.interior > s->max_active_cells) break;
+        // The cap holds back this engine only: later requests of other engines may pass.
+        if (s->max_active_cells > 0 && s->running[e] > 0 && s->active_cells[e] + r.interior > s->max_active_cells) {
+            ++q;
+            continue;
+        }
         size_t off = 0;
         if (!s->arena.alloc(r.bytes, off)) break;
         char* p = s->base + off;
@@ -214,7 +217,7 @@ int admit(hydro_server* s)
         r.t_admit = s->now();
         s->slot_req[e][slot] = id;
         ++s->running[e];
-        s->active_cells += r.interior;
+        s->active_cells[e] += r.interior;
         q = s->queue.erase(q);
     }
     return 0;
@@ -247,7 +250,7 @@ int retire(hydro_server* s, int e, size_t* n_done)
             if (rc) return rc;
             s->slot_req[e][slot] = -1;
             --s->running[e];
-            s->active_cells -= r.interior;
+            s->active_cells[e] -= r.interior;
             if (!r.keep || r.state == kFailed) release_memory(s, r);
             if (n_done) ++*n_done;
         }
