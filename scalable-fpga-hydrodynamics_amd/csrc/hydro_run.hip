@@ -3,10 +3,13 @@
 // hydro_run moves the host loop of hydro2d/main.cxx:686-815 (hydro3d/main.cxx:742-876)
 // onto the device: Dt, t and the iteration count live in a StepControl block, each
 // step's kernels read Dt from it (DtDx = Dt / DX computed on the device, the same IEEE
-// division the host performs) and the workgroup that finalises the Dt reduction
-// records (t, Dt), advances t += Dt and sets `done` when the loop condition fails.  The
-// host can therefore enqueue many steps back to back (or capture them in a graph)
-// without reading Dt between steps.
+// division the host performs), and the one-wave finalise kernel that follows each step
+// kernel turns the speed reduction into the next Dt, records (t, Dt), advances
+// t += Dt and sets `done` when the loop condition fails.  The host can therefore enqueue
+// many steps back to back (or capture them in a graph) without reading Dt between
+// steps.  A step is the step kernel and the finalise kernel; periodic axes are read from
+// the opposite interior plane inside the step kernel, so only a reflective X boundary
+// adds a ghost-fill launch.
 #include <hip/hip_runtime.h>
 
 #include <new>
@@ -285,11 +288,11 @@ extern "C" void hydro_run_destroy(hydro_run* r)
 }
 
 // ---------------------------------------------------------------------------------
-// Total mass: sum of interior rho with error-free two-sum accumulation (each lane keeps
-// a (sum, error) pair; pairs are merged in a tree), then one atomic-free finish by the
-// last workgroup.  The reference sums on the host in long double with Neumaier
-// compensation (hydro2d/main.cxx:237-265); both are accurate far below the printed
-// 8 significant digits.
+// Total mass (and energy): sum of interior rho (E) with error-free two-sum accumulation
+// (each lane keeps a (sum, error) pair; pairs are merged in a tree per workgroup), then
+// a one-thread finish kernel over the workgroups' pairs.  The reference sums on the
+// host in long double with Neumaier compensation (hydro2d/main.cxx:237-265); both are
+// accurate far below the printed 8 significant digits.
 // ---------------------------------------------------------------------------------
 namespace {
 
