@@ -413,6 +413,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="2d")
     ap.add_argument("--rows", type=int, default=0, help="override the kernel's X rows per strip")
+    ap.add_argument("--occ", type=int, default=0, help="override the 2D kernel's workgroups per CU (3..8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-reference", action="store_true", help="skip the achievable-bandwidth copy")
     ap.add_argument("--no-forms", action="store_true",
@@ -469,6 +470,8 @@ def main() -> int:
     hydro.lib()
     if args.rows:
         hydro.tune_set("rows2d" if dim == 2 else "rows3d", args.rows)
+    if args.occ and dim == 2:
+        hydro.tune_set("occ2d", args.occ)
 
     fp32_state = precision == 32
     tdt = torch.float32 if fp32_state else torch.float64
@@ -573,7 +576,8 @@ def main() -> int:
         "data": "synthetic (built-in blast IC generated on the device)",
         "config": {"workload": name, "grid": list(nb), "cells": cells, "boundaries": "periodic X, reflective Y/Z",
                    "steps_timed": K, "kernel": "fvm2d" if dim == 2 else "fvm3d",
-                   "rows_per_strip": hydro.tune_get("rows2d" if dim == 2 else "rows3d")},
+                   "rows_per_strip": hydro.tune_get("rows2d_last" if dim == 2 else "rows3d_last"),
+                   "workgroups_per_cu": hydro.tune_get("occ2d_last") if dim == 2 else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(name)},
         "cpu_baseline": cpu,

@@ -23,6 +23,9 @@
 // Dt = C_min / max_speed and clears the slots for the next step.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <type_traits>
+
 #include "../../include/hydro_hip.h"
 #include "hydro_device.hpp"
 #include "hydro_kernels.hpp"
@@ -142,10 +145,45 @@ __global__ void ghost_y2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long 
     M2[dst] = M2[src];
 }
 
-int g_rows_2d = 64;  // maximum rows per strip (tunable, hydro_tune_set)
+// Launch geometry (no effect on results): maximum rows per strip and workgroups per CU,
+// set by hydro_tune_set; 0 selects the per-precision defaults below.  The geometry of
+// the last launch (after the strip adaptation) is kept for reporting.
+std::atomic<int> g_rows_2d{0}, g_occ_2d{0};
+std::atomic<int> g_last_rows_2d{0}, g_last_occ_2d{0};
+
+// Defaults from interleaved sweeps of the 40000x20000 blast on MI355X (strips of 8..64
+// rows x 3..8 workgroups per CU, two boxes): short strips and fewer waves in flight keep
+// fewer DRAM rows open at once.  16-row strips with 4 (FP64), 3 (FP32-flux) or 5 (FP32)
+// workgroups per CU run 7-19% faster than 64-row strips at full occupancy.
+constexpr int kDefaultRows2d = 16;
+template <typename T, typename TF>
+constexpr int default_occ_2d()
+{
+    if constexpr (std::is_same_v<T, float>) return 5;
+    if constexpr (std::is_same_v<TF, float>) return 3;
+    return 4;
+}
+
 // Workgroups a step should have to fill the chip (8 per CU on 256 CUs); smaller grids
 // use shorter strips (down to 8 rows) until they have that many.
 constexpr long long kFillBlocks2 = 2048;
+
+// Dynamic LDS that leaves room for exactly `occ` workgroups of Kernel per CU (160 KiB of
+// LDS): the middle of the interval (L/(occ+1), L/occ], less the kernel's static LDS, so
+// allocation granularity cannot move the count.  Fewer waves in flight per CU keep fewer
+// DRAM rows open at once; the step kernels need no LDS of their own for this.
+template <auto Kernel>
+size_t occupancy_lds(int occ)
+{
+    if (occ <= 0) return 0;
+    static const size_t static_lds = [] {
+        hipFuncAttributes fa;
+        return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(Kernel)) == hipSuccess ? fa.sharedSizeBytes
+                                                                                              : size_t(0);
+    }();
+    const size_t total = (kLdsPerCu / (occ + 1) + kLdsPerCu / occ) / 2;
+    return total > static_lds ? total - static_lds : 0;
+}
 
 template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
@@ -172,7 +210,10 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     g.nwb = (g.nwin + kWavesPerBlock - 1) / kWavesPerBlock;
     g.wrap_x = (wraps & kWrapX) ? 1 : 0;
     g.wrap_y = (wraps & kWrapY) ? 1 : 0;
-    g.rows = g_rows_2d;
+    const int rows_set = g_rows_2d.load(std::memory_order_relaxed);
+    const int occ_set = g_occ_2d.load(std::memory_order_relaxed);
+    const int occ = occ_set > 0 ? occ_set : default_occ_2d<T, TF>();
+    g.rows = rows_set > 0 ? rows_set : kDefaultRows2d;
     while (g.rows > 8 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nwb < kFillBlocks2) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
     const long long nblocks = nstrips * g.nwb;
@@ -182,21 +223,27 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     }
     const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock);
     const bool wall = bc_y == HYDRO_BC_REFLECTIVE;
+    g_last_rows_2d.store(g.rows, std::memory_order_relaxed);
+    g_last_occ_2d.store(occ, std::memory_order_relaxed);
     if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);
+    auto go = [&](auto wall_c, auto resident_c) {
+        constexpr auto kernel = fvm2d_kernel<T, TF, M, decltype(wall_c)::value, decltype(resident_c)::value>;
+        const T dx = decltype(resident_c)::value ? T(0) : DtDx, dy = decltype(resident_c)::value ? T(0) : DtDy;
+        hipLaunchKernelGGL(kernel, grid, block, occupancy_lds<kernel>(occ), s, rhoE, uv, rhoE_next, uv_next, g, c,
+                           dx, dy, red, ctl, Dt_next, traj_t, traj_dt);
+    };
+    using yes = std::true_type;
+    using no = std::false_type;
     if (ctl) {
         if (wall)
-            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, true, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
-                               T(0), T(0), red, ctl, Dt_next, traj_t, traj_dt);
+            go(yes{}, yes{});
         else
-            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, false, true>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
-                               T(0), T(0), red, ctl, Dt_next, traj_t, traj_dt);
+            go(no{}, yes{});
     } else {
         if (wall)
-            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, true, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
-                               DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
+            go(yes{}, no{});
         else
-            hipLaunchKernelGGL((fvm2d_kernel<T, TF, M, false, false>), grid, block, 0, s, rhoE, uv, rhoE_next, uv_next, g, c,
-                               DtDx, DtDy, red, ctl, Dt_next, traj_t, traj_dt);
+            go(no{}, no{});
     }
     HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
     if (ev_b) HYDRO_HIP_OR(hipEventRecord(ev_b, s), HYDRO_ERR_HIP);
@@ -328,12 +375,19 @@ int ghost2d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, int bc_x, int b
     return launch_ghost2d<float>(static_cast<float*>(rhoE), static_cast<float*>(mom), nb_x, nb_y, bc_x, bc_y, ctl, s);
 }
 
-void set_rows_2d(int rows)
+void set_rows_2d(int rows) { g_rows_2d.store(rows, std::memory_order_relaxed); }
+
+int get_rows_2d(bool last)
 {
-    if (rows >= 4 && rows <= 4096) g_rows_2d = rows;
+    return (last ? g_last_rows_2d : g_rows_2d).load(std::memory_order_relaxed);
 }
 
-int get_rows_2d() { return g_rows_2d; }
+void set_occ_2d(int occ) { g_occ_2d.store(occ, std::memory_order_relaxed); }
+
+int get_occ_2d(bool last)
+{
+    return (last ? g_last_occ_2d : g_occ_2d).load(std::memory_order_relaxed);
+}
 
 }  // namespace hydro
 

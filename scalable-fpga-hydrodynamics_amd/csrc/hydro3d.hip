@@ -18,6 +18,9 @@
 // are consecutive in Z, so every wave row is one contiguous stream per array.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <type_traits>
+
 #include "../../include/hydro_hip.h"
 #include "hydro_device.hpp"
 #include "hydro_kernels.hpp"
@@ -174,7 +177,12 @@ __global__ void ghost_z3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long
     M3[dst] = M3[src];
 }
 
-int g_rows_3d = 32;  // maximum X rows per strip (tunable, hydro_tune_set)
+// Maximum X rows per strip (no effect on results), set by hydro_tune_set; 0 selects the
+// per-precision default: 32 rows, 16 for FP32-flux (sweeps of 4..64 rows on the 512^3
+// blast: FP64 is fastest at 32, FP32-flux 3% faster at 8..16 than at 32).  The rows of
+// the last launch (after the strip adaptation) are kept for reporting.
+std::atomic<int> g_rows_3d{0};
+std::atomic<int> g_last_rows_3d{0};
 // Workgroups a step should have to fill the chip (4 per CU); smaller grids use shorter
 // strips (down to 4 rows) until they have that many.
 constexpr long long kFillBlocks3 = 1024;
@@ -209,7 +217,8 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     g.wrap_x = (wraps & kWrapX) ? 1 : 0;
     g.wrap_y = (wraps & kWrapY) ? 1 : 0;
     g.wrap_z = (wraps & kWrapZ) ? 1 : 0;
-    g.rows = g_rows_3d;
+    const int rows_set = g_rows_3d.load(std::memory_order_relaxed);
+    g.rows = rows_set > 0 ? rows_set : (std::is_same_v<T, double> && std::is_same_v<TF, float> ? 16 : 32);
     while (g.rows > 4 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nzw * g.nyw < kFillBlocks3) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
     const long long nblocks = nstrips * g.nzw * g.nyw;
@@ -219,6 +228,7 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     }
     const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock3);
     const bool wy = bc_y == HYDRO_BC_REFLECTIVE, wz = bc_z == HYDRO_BC_REFLECTIVE;
+    g_last_rows_3d.store(g.rows, std::memory_order_relaxed);
     if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);
 #define HYDRO_L3(WY, WZ, RES)                                                                                 \
     hipLaunchKernelGGL((fvm3d_kernel<T, TF, M, WY, WZ, RES>), grid, block, 0, s, rhoE, uvw, rhoE_next, uvw_next, g, c, \
@@ -374,12 +384,9 @@ int ghost3d_any(void* rhoE, void* mom, size_t nb_x, size_t nb_y, size_t nb_z, in
                                  bc_z, ctl, s);
 }
 
-void set_tile_3d(int rows)
-{
-    if (rows >= 2 && rows <= 4096) g_rows_3d = rows;
-}
+void set_tile_3d(int rows) { g_rows_3d.store(rows, std::memory_order_relaxed); }
 
-int get_tile_3d() { return g_rows_3d; }
+int get_tile_3d(bool last) { return (last ? g_last_rows_3d : g_rows_3d).load(std::memory_order_relaxed); }
 
 }  // namespace hydro
 

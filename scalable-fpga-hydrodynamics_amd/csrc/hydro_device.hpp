@@ -20,6 +20,7 @@
 namespace hydro {
 
 constexpr int kWave = 64;
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950 LDS per CU
 
 // Sign bit of the (high word of the) floating-point type.
 template <typename T>

@@ -35,9 +35,13 @@ int ic_blast_async_f64(double* d_rhoE, double* d_mom, int dim, const size_t* nb,
                        double p_in, double p_out, int ic_mode, double radius_frac, double C_min, double gamma,
                        unsigned long long* d_flags, double* dt_in, double* dt_out, hipStream_t s);
 
+// Launch geometry knobs (0 = per-precision default); get_*(true) returns the geometry of
+// the last launch after the strip adaptation.
 void set_rows_2d(int rows);
+void set_occ_2d(int occ);
 void set_tile_3d(int rows);
-int get_rows_2d();
-int get_tile_3d();
+int get_rows_2d(bool last);
+int get_occ_2d(bool last);
+int get_tile_3d(bool last);
 
 }  // namespace hydro

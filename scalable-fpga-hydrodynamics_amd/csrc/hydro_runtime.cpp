@@ -48,11 +48,16 @@ extern "C" int hydro_version(void) { return HYDRO_HIP_ABI_VERSION; }
 extern "C" int hydro_tune_set(const char* key, int value)
 {
     hydro::clear_error();
-    if (key && strcmp(key, "rows2d") == 0 && value >= 4 && value <= 4096) {
+    const bool is = key != nullptr;
+    if (is && strcmp(key, "rows2d") == 0 && (value == 0 || (value >= 4 && value <= 4096))) {
         hydro::set_rows_2d(value);
         return 0;
     }
-    if (key && strcmp(key, "rows3d") == 0 && value >= 2 && value <= 4096) {
+    if (is && strcmp(key, "occ2d") == 0 && (value == 0 || (value >= 3 && value <= 8))) {
+        hydro::set_occ_2d(value);
+        return 0;
+    }
+    if (is && strcmp(key, "rows3d") == 0 && (value == 0 || (value >= 2 && value <= 4096))) {
         hydro::set_tile_3d(value);
         return 0;
     }
@@ -62,7 +67,12 @@ extern "C" int hydro_tune_set(const char* key, int value)
 
 extern "C" int hydro_tune_get(const char* key)
 {
-    if (key && strcmp(key, "rows2d") == 0) return hydro::get_rows_2d();
-    if (key && strcmp(key, "rows3d") == 0) return hydro::get_tile_3d();
+    if (!key) return HYDRO_ERR_ARG;
+    if (strcmp(key, "rows2d") == 0) return hydro::get_rows_2d(false);
+    if (strcmp(key, "occ2d") == 0) return hydro::get_occ_2d(false);
+    if (strcmp(key, "rows3d") == 0) return hydro::get_tile_3d(false);
+    if (strcmp(key, "rows2d_last") == 0) return hydro::get_rows_2d(true);
+    if (strcmp(key, "occ2d_last") == 0) return hydro::get_occ_2d(true);
+    if (strcmp(key, "rows3d_last") == 0) return hydro::get_tile_3d(true);
     return HYDRO_ERR_ARG;
 }

@@ -71,11 +71,19 @@ def test_bad_arguments_are_rejected_without_a_device(hydro):
 
 
 def test_tuning_knobs_round_trip(hydro):
-    old = hydro.tune_get("rows2d")
-    hydro.tune_set("rows2d", 96)
-    assert hydro.tune_get("rows2d") == 96
-    hydro.tune_set("rows2d", old)
-    assert hydro.tune_get("rows3d") >= 2
+    for key, value, bad in [("rows2d", 96, 3), ("occ2d", 3, 2), ("occ2d", 8, 9), ("rows3d", 7, 1)]:
+        old = hydro.tune_get(key)
+        assert old == 0  # per-precision defaults
+        hydro.tune_set(key, value)
+        assert hydro.tune_get(key) == value
+        with pytest.raises(hydro.HydroError):
+            hydro.tune_set(key, bad)
+        assert hydro.tune_get(key) == value
+        hydro.tune_set(key, old)
+        assert hydro.tune_get(key) == 0
+    # No step launched in this process yet: the last geometry reads 0.
+    assert hydro.tune_get("rows2d_last") >= 0
+    assert hydro.tune_get("no_such_knob") == -1
 
 
 def test_missing_library_fails_loudly(tmp_path):

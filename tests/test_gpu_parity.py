@@ -223,6 +223,61 @@ def test_resident_run3d(gpu, oracle, nb, bc):
     assert_bitwise(interior3(gu), interior3(uvw), "uvw after run")
 
 
+def _geometry_run(gpu, nb, precision, steps, knobs):
+    """Blast IC on the device, `steps` resident steps under the given tune knobs; returns
+    the final state on the host and the geometry the last launch used."""
+    fp32 = precision == 32
+    k = gpu.constants(nb, 0.4, "f32" if fp32 else "f64")
+    tdt = torch.float32 if fp32 else torch.float64
+    shape = tuple(n + 2 for n in nb)
+    a = [torch.zeros(shape + (2,), dtype=tdt, device="cuda"), torch.zeros(shape + (len(nb),), dtype=tdt, device="cuda")]
+    b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+    dt0 = gpu.ic(a[0], a[1], k["C_min"])
+    old = {key: gpu.tune_get(key) for key in knobs}
+    try:
+        for key, v in knobs.items():
+            gpu.tune_set(key, v)
+        run = gpu.Run(nb, k["D"], k["C_min"], dt0, 1e30, steps, precision, [0, 1, 1][: len(nb)])
+        run.advance(a, b, steps)
+        st = run.status()
+        assert st["rc"] == 0 and st["it"] == steps, st
+        last = {key: gpu.tune_get(key + "_last") for key in knobs}
+        run.close()
+    finally:
+        for key, v in old.items():
+            gpu.tune_set(key, v)
+    cur = a if st["current"] == 0 else b
+    return (to_host(cur[0]), to_host(cur[1])), last
+
+
+@pytest.mark.parametrize("precision", [64, 3264, 32])
+def test_launch_geometry_keeps_results_2d(gpu, precision):
+    """Strip rows and workgroups per CU change the launch, never the result: a grid tall
+    enough that 64-row strips are not shortened, ragged strips (37 rows), and 3..8
+    workgroups per CU, bitwise against the default geometry (itself pinned to the oracle
+    by the other tests)."""
+    nb = (14601, 2000)
+    base, last = _geometry_run(gpu, nb, precision, 8, {"rows2d": 0, "occ2d": 0})
+    assert last == {"rows2d": 16, "occ2d": {64: 4, 3264: 3, 32: 5}[precision]}
+    for rows, occ in [(64, 8), (37, 3), (4, 6)]:
+        got, last = _geometry_run(gpu, nb, precision, 8, {"rows2d": rows, "occ2d": occ})
+        assert last == {"rows2d": rows, "occ2d": occ}
+        assert_bitwise(got[0], base[0], f"rhoE rows {rows} occ {occ}")
+        assert_bitwise(got[1], base[1], f"uv rows {rows} occ {occ}")
+
+
+@pytest.mark.parametrize("precision", [64, 3264])
+def test_launch_geometry_keeps_results_3d(gpu, precision):
+    nb = (300, 256, 256)
+    base, last = _geometry_run(gpu, nb, precision, 6, {"rows3d": 0})
+    assert last == {"rows3d": 16 if precision == 3264 else 32}
+    for rows in (64, 7, 2):
+        got, last = _geometry_run(gpu, nb, precision, 6, {"rows3d": rows})
+        assert last == {"rows3d": rows}
+        assert_bitwise(got[0], base[0], f"rhoE rows {rows}")
+        assert_bitwise(got[1], base[1], f"uvw rows {rows}")
+
+
 def test_total_mass_device(gpu, oracle):
     rhoE, uv, _ = oracle.ic2d(300, 170)
     m = gpu.total_mass(to_dev(rhoE))
