@@ -54,8 +54,12 @@ struct Geom2 {
 // fluxes and signal speeds.  <double, double, IeeeMath> is the reference FP64 scheme,
 // <float, float, IeeeMath> its USE_FLOAT build, <double, float, FastMath> the FP32-flux
 // form (FP32 fluxes, increments and update accumulated in FP64).
+template <typename TS, typename TF>
+constexpr int kMinBlocks2 =
+    std::is_same_v<TS, double> && std::is_same_v<TF, double> ? HYDRO_2D_MIN_BLOCKS_FP64 : HYDRO_2D_MIN_BLOCKS;
+
 template <typename TS, typename TF, typename M, bool WALL_Y, bool RESIDENT>
-__global__ void __launch_bounds__(kBlock, HYDRO_2D_MIN_BLOCKS)
+__global__ void __launch_bounds__(kBlock, (kMinBlocks2<TS, TF>))
     fvm2d_kernel(const TS* __restrict__ rhoE, const TS* __restrict__ uv, TS* __restrict__ rhoE_next,
                  TS* __restrict__ uv_next, Geom2 g, Consts<TS> cs, TS DtDx_arg, TS DtDy_arg, StepReduce* red,
                  StepControl* ctl, TS* Dt_next, double* traj_t, TS* traj_dt)
@@ -152,10 +156,15 @@ std::atomic<int> g_rows_2d{0}, g_occ_2d{0};
 std::atomic<int> g_last_rows_2d{0}, g_last_occ_2d{0};
 
 // Defaults from interleaved sweeps of the 40000x20000 blast on MI355X (strips of 8..64
-// rows x 3..8 workgroups per CU, two boxes): short strips and fewer waves in flight keep
-// fewer DRAM rows open at once.  16-row strips with 4 (FP64), 3 (FP32-flux) or 5 (FP32)
-// workgroups per CU run 7-19% faster than 64-row strips at full occupancy.
-constexpr int kDefaultRows2d = 16;
+// rows x 3..8 workgroups per CU, three boxes): short strips and fewer waves in flight
+// keep fewer DRAM rows open at once.  12-row strips with 4 (FP64) or 3 (FP32-flux)
+// workgroups per CU and 16-row strips with 5 (FP32) run 7-19% faster than 64-row
+// strips at full occupancy.
+template <typename T, typename TF>
+constexpr int default_rows_2d()
+{
+    return std::is_same_v<T, float> ? 16 : 12;
+}
 template <typename T, typename TF>
 constexpr int default_occ_2d()
 {
@@ -213,7 +222,7 @@ int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Co
     const int rows_set = g_rows_2d.load(std::memory_order_relaxed);
     const int occ_set = g_occ_2d.load(std::memory_order_relaxed);
     const int occ = occ_set > 0 ? occ_set : default_occ_2d<T, TF>();
-    g.rows = rows_set > 0 ? rows_set : kDefaultRows2d;
+    g.rows = rows_set > 0 ? rows_set : default_rows_2d<T, TF>();
     while (g.rows > 8 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nwb < kFillBlocks2) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
     const long long nblocks = nstrips * g.nwb;

@@ -24,8 +24,13 @@ namespace hydro {
 
 constexpr int kOwned = kWave - 2;  // owned columns per wave window
 constexpr int kWavesPerBlock = 4;
+// Workgroups per CU the 2D step must fit in registers: 5 (at most 102 VGPRs), and 4 for
+// the FP64 scheme (102 VGPRs; at 5 it spills 6 registers, 7% slower).
 #ifndef HYDRO_2D_MIN_BLOCKS
 #define HYDRO_2D_MIN_BLOCKS 5
+#endif
+#ifndef HYDRO_2D_MIN_BLOCKS_FP64
+#define HYDRO_2D_MIN_BLOCKS_FP64 4
 #endif
 // Rows in flight ahead of the row being converted (register prefetch ring depth).
 #ifndef HYDRO_2D_PREFETCH
