@@ -335,18 +335,28 @@ int build_tables(hydro_batch* b)
             }
         }
     } else {
+        // Tiles in row-major order within each blast (all columns of tile row 0, then tile
+        // row 1, ...), dealt round-robin: the workers running at the same time hold
+        // horizontally adjacent windows of the same rows, as the plain step's workgroups
+        // do, so halo columns and DRAM pages are shared while they are in L2.  Column-major
+        // order (each column's tiles in turn) ran 6% slower on the 256-blast mix.
         const int R = b->tile_rows;
         size_t t = 0;
-        for (const Col& c : cols)
-            for (int i0 = 1; i0 <= c.rows; i0 += R) {
-                BatchSeg sg;
-                sg.blast = c.blast;
-                sg.col = c.col;
-                sg.i0 = i0;
-                sg.i1 = std::min(i0 + R - 1, c.rows);
-                per[t % W].push_back(sg);
-                ++t;
-            }
+        for (size_t c0 = 0; c0 < cols.size();) {
+            size_t c1 = c0;
+            while (c1 < cols.size() && cols[c1].blast == cols[c0].blast) ++c1;
+            for (int i0 = 1; i0 <= cols[c0].rows; i0 += R)
+                for (size_t ci = c0; ci < c1; ++ci) {
+                    BatchSeg sg;
+                    sg.blast = cols[ci].blast;
+                    sg.col = cols[ci].col;
+                    sg.i0 = i0;
+                    sg.i1 = std::min(i0 + R - 1, cols[ci].rows);
+                    per[t % W].push_back(sg);
+                    ++t;
+                }
+            c0 = c1;
+        }
     }
     b->h_segs.clear();
     b->h_off.assign(W + 1, 0u);
