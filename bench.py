@@ -528,7 +528,7 @@ def main() -> int:
     # Achievable bandwidth: the library's stream copy over the same number of bytes one
     # step reads (and writes), on buffers of the state's size.  Reuses the state's
     # second buffer pair as the destination after the run.
-    copy_gbs = None
+    copy_gbs = read_gbs = write_gbs = None
     if not args.no_copy_reference:
         src, dst = cur[0], (b if st["current"] == 0 else a)[0]
         hydro.stream_copy(dst, src)
@@ -541,7 +541,25 @@ def main() -> int:
         torch.cuda.synchronize()
         nbytes = src.numel() * src.element_size()
         copy_gbs = 2 * nbytes / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
-        log(f"[bench] stream copy {nbytes / 1e9:.1f} GB: {copy_gbs:.0f} GB/s")
+        # Each direction alone over the same buffers: a read-only stream (non-temporal
+        # loads) and a write-only fill.
+        slots = torch.zeros(256, dtype=torch.int64, device="cuda")
+        hydro.stream_read(src, slots)
+        torch.cuda.synchronize()
+        c0.record()
+        for _ in range(3):
+            hydro.stream_read(src, slots)
+        c1.record()
+        torch.cuda.synchronize()
+        read_gbs = nbytes / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
+        c0.record()
+        for _ in range(3):
+            hydro.stream_fill(dst, 0x3FF0000000000000)
+        c1.record()
+        torch.cuda.synchronize()
+        write_gbs = nbytes / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
+        log(f"[bench] stream {nbytes / 1e9:.1f} GB: copy {copy_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s, "
+            f"write {write_gbs:.0f} GB/s")
 
     if args.calibrate_copy:
         # A plain copy of exactly 1 GiB (read 1 GiB, write 1 GiB): what the FETCH_SIZE /
@@ -583,6 +601,8 @@ def main() -> int:
         "cpu_baseline": cpu,
         "hbm_frac_whole_step": round(bpc * cells * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "achievable_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+        "achievable_read_gbs": round(read_gbs, 1) if read_gbs else None,
+        "achievable_write_gbs": round(write_gbs, 1) if write_gbs else None,
         "frac_of_achievable": round(achieved / copy_gbs, 4) if copy_gbs else None,
         "kernel_ms": round(avg_kernel_s * 1e3, 4),
         "energy": ({"joules_per_step": round((j1 - j0) / K, 4), "nj_per_cell_update": round((j1 - j0) / K / cells * 1e9, 4),

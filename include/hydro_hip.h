@@ -321,6 +321,14 @@ int hydro_total_energy(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z
  * the state once.  No counterpart in the reference (measurement only). */
 int hydro_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* Read-only and write-only streams over `bytes` (a multiple of 16): the achievable
+ * bandwidth of each direction alone.  hydro_stream_read XOR-folds every 64-bit word of
+ * src into the 256 slots of d_xor256 (device memory the caller zeroes first; the XOR of
+ * the 256 slots is the XOR of all words).  hydro_stream_fill writes `pattern` to every
+ * 64-bit word of dst.  No counterpart in the reference (measurement only). */
+int hydro_stream_read(const void* src, size_t bytes, unsigned long long* d_xor256, void* stream);
+int hydro_stream_fill(void* dst, size_t bytes, unsigned long long pattern, void* stream);
+
 const char* hydro_last_error(void);
 int hydro_version(void);
 

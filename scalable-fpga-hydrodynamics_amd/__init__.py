@@ -87,6 +87,8 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_total_energy.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_stream_copy.argtypes = [vp, vp, sz, vp]
+    L.hydro_stream_read.argtypes = [vp, sz, vp, vp]
+    L.hydro_stream_fill.argtypes = [vp, sz, C.c_ulonglong, vp]
     L.hydro_batch_create.argtypes = [i32, i32, sz]
     L.hydro_batch_create.restype = vp
     L.hydro_batch_destroy.argtypes = [vp]
@@ -511,8 +513,24 @@ def stream_copy(dst, src, stream=None) -> None:
     _check(lib().hydro_stream_copy(_p(dst), _p(src), nbytes, _stream(stream)), "hydro_stream_copy")
 
 
+def stream_read(src, slots, stream=None) -> None:
+    """Read-only stream over src: XOR of every 64-bit word folded into `slots` (256
+    uint64 on the device, zeroed by the caller)."""
+    nbytes = src.numel() * src.element_size()
+    if slots.numel() * slots.element_size() < 256 * 8:
+        raise HydroError("stream_read: slots must hold 256 64-bit words")
+    _check(lib().hydro_stream_read(_p(src), nbytes, _p(slots), _stream(stream)), "hydro_stream_read")
+
+
+def stream_fill(dst, pattern: int, stream=None) -> None:
+    """Write-only stream: the 64-bit `pattern` to every word of dst."""
+    nbytes = dst.numel() * dst.element_size()
+    _check(lib().hydro_stream_fill(_p(dst), nbytes, int(pattern) & (2**64 - 1), _stream(stream)), "hydro_stream_fill")
+
+
 def tune_set(key: str, value: int) -> None:
-    """Launch-geometry knobs ("rows2d", "rows3d"); results do not depend on them."""
+    """Launch-geometry knobs ("rows2d", "occ2d", "rows3d"; 0 = per-precision default);
+    results do not depend on them."""
     _check(lib().hydro_tune_set(key.encode(), int(value)), f"hydro_tune_set({key})")
 
 

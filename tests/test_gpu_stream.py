@@ -1,5 +1,5 @@
-"""The stream-copy reference kernel copies every byte exactly, for sizes that end inside
-a block's chunk, on a chunk boundary, and below one block."""
+"""The stream reference kernels (copy, read, fill) touch every byte exactly once, for
+sizes that end inside a block's chunk, on a chunk boundary, and below one block."""
 import pytest
 import torch
 
@@ -15,4 +15,30 @@ def test_stream_copy_exact(gpu, n16):
     torch.cuda.synchronize()
     assert torch.equal(dst[: 2 * n16], src[: 2 * n16])
     # Nothing past the end is written.
+    assert torch.equal(dst[2 * n16:], torch.full((2,), -1, dtype=torch.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("n16", [1, 7, 16384, 16384 + 5, 3 * 16384, 5 * 16384 - 1, 1 << 20])
+def test_stream_read_folds_every_word(gpu, n16):
+    """The read stream's 256 XOR slots fold to the XOR of every 64-bit word of the input
+    (computed on the host), so every byte is read once."""
+    import numpy as np
+
+    g = torch.Generator(device="cuda").manual_seed(1000 + n16)
+    src = torch.randint(-(2**62), 2**62, (2 * n16,), dtype=torch.int64, device="cuda", generator=g)
+    slots = torch.zeros(256, dtype=torch.int64, device="cuda")
+    gpu.stream_read(src, slots)
+    torch.cuda.synchronize()
+    got = np.bitwise_xor.reduce(slots.cpu().numpy().view(np.uint64))
+    want = np.bitwise_xor.reduce(src.cpu().numpy().view(np.uint64))
+    assert got == want
+
+
+@pytest.mark.parametrize("n16", [1, 7, 16384 + 5, 5 * 16384 - 1])
+def test_stream_fill_exact(gpu, n16):
+    dst = torch.full((2 * n16 + 2,), -1, dtype=torch.int64, device="cuda")
+    pattern = 0x3FF0_0000_0000_0001
+    gpu.stream_fill(dst[: 2 * n16], pattern)
+    torch.cuda.synchronize()
+    assert torch.equal(dst[: 2 * n16], torch.full((2 * n16,), pattern, dtype=torch.int64, device="cuda"))
     assert torch.equal(dst[2 * n16:], torch.full((2,), -1, dtype=torch.int64, device="cuda"))
