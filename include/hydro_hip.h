@@ -190,9 +190,11 @@ int hydro_ic_f32(float* d_rhoE, float* d_mom, int dim, const size_t* nb, const s
  * every active blast, followed by one finalise launch that computes each blast's Dt,
  * advances its t / iteration count and flips its current buffer; a blast stops on its
  * own when !(t < max_t && it < max_it).  Steps may be replayed from a hipGraph.  The
- * batch runs on its own non-blocking stream (hydro_batch_stream); buffers handed to it
- * must be ready before the first advance (synchronise the producing stream).  A batch
- * is used from one host thread at a time; distinct batches may run concurrently.
+ * batch runs on its own non-blocking stream (hydro_batch_stream): work queued on a
+ * blast's buffers in other streams must be complete, or ordered before that stream
+ * (hipStreamWaitEvent), when hydro_batch_add / hydro_batch_add_blast is called, since
+ * add_blast writes the initial state on the batch stream at once.  A batch is used
+ * from one host thread at a time; distinct batches may run concurrently.
  * ------------------------------------------------------------------------------- */
 
 typedef struct hydro_batch hydro_batch;

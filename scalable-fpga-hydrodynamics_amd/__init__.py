@@ -348,8 +348,16 @@ class Batch:
     def stream(self) -> int:
         return lib().hydro_batch_stream(self._h) or 0
 
+    def _after_current_stream(self) -> None:
+        """The batch works on its own non-blocking stream: order it after the work already
+        queued on torch's current stream (the fills and copies of the buffers being added)."""
+        import torch
+
+        torch.cuda.ExternalStream(self.stream).wait_stream(torch.cuda.current_stream())
+
     def add(self, a, b, D, C_min, dt0, max_t, max_it) -> int:
         nb = tuple(n - 2 for n in a[0].shape[: self.dim])
+        self._after_current_stream()
         slot = lib().hydro_batch_add(self._h, _sizes(nb), (C.c_double * 3)(*[float(x) for x in D] + [0.0] * (3 - len(D))),
                                      float(C_min), float(dt0), float(max_t), int(max_it), _p(a[0]), _p(a[1]), _p(b[0]),
                                      _p(b[1]))
@@ -361,6 +369,7 @@ class Batch:
     def add_blast(self, a, b, C, radius_frac, p_in, p_out=0.1, rho_in=1.0, rho_out=1.2, max_t=1e30,
                   max_it=100) -> int:
         nb = tuple(n - 2 for n in a[0].shape[: self.dim])
+        self._after_current_stream()
         slot = lib().hydro_batch_add_blast(self._h, _sizes(nb), float(C), float(radius_frac), float(rho_in),
                                            float(rho_out), float(p_in), float(p_out), float(max_t), int(max_it),
                                            _p(a[0]), _p(a[1]), _p(b[0]), _p(b[1]))

@@ -83,6 +83,24 @@ def test_batch2d_matches_oracle_per_blast(gpu, oracle):
     batch.close()
 
 
+def test_batch_add_orders_after_pending_buffer_work(gpu, oracle):
+    """add_blast writes the initial state on the batch's own non-blocking stream at once.
+    Work still queued on the buffers in torch's stream (here some 160 zero fills) must
+    finish first, or it would overwrite the initial state."""
+    nb = (2048, 1024)
+    a, b = _bufs(nb)
+    for _ in range(40):
+        for t in a + b:
+            t.zero_()
+    batch = gpu.Batch(2, gpu.FP64, 1)
+    rf, p_in = 0.15, 10.0
+    slot = batch.add_blast(a, b, 0.5, rf, p_in, max_it=5)
+    batch.advance(5)
+    batch.sync()
+    _check_slot(batch, slot, nb, _oracle(oracle, nb, 0.5, rf, p_in, 5), "after queued fills")
+    batch.close()
+
+
 def test_batch2d_tiling_changes_mid_run(gpu, oracle):
     batch, slots = _make(gpu, 2, SIZES_2D, 30)
     for bpc, rows, width in [(1, 0, 1), (4, 32, 2), (2, 128, 4), (3, 0, 4), (2, 0, 2)]:
