@@ -234,11 +234,17 @@ int hydro_batch_status(hydro_batch* batch, int slot, size_t* it, double* t, doub
  * non-finite Dt (waits for the batch stream). */
 int hydro_batch_done_flags(hydro_batch* batch, unsigned char* flags, size_t n);
 
-/* Tiling: workgroups per CU of the persistent grid (0 = default), tile rows (0 =
- * balanced: equal contiguous row ranges per worker; R > 0: R-row tiles dealt
- * round-robin) and, in 2D, tile width in 62-column windows (1, 2 or 4 adjacent windows
- * marched together by one worker of as many waves; 0 = 1).  Results do not depend on
- * the tiling. */
+/* Tiling: workgroups per CU (0 = default), tile rows (0 = balanced: equal contiguous
+ * row ranges per worker of a persistent grid; R > 0: tiles of R rows in row-major order
+ * within each blast) and, in 2D, tile width in 62-column windows (1, 2 or 4 adjacent
+ * windows marched together by one worker of as many waves; 0 = 1).
+ *   Persistent grid (balanced, multi-window tiles, 3D): blocks_per_cu sizes the grid
+ *   and the tiles are dealt round-robin to its workers.
+ *   Direct dispatch (2D, R > 0, one window): one wave per tile in table order;
+ *   blocks_per_cu caps the resident workgroups per CU (values below 3 act as 3; 0 = as
+ *   registers allow), and R is halved, down to 8, until the step has at least 2048
+ *   workgroups.
+ * Results do not depend on the tiling. */
 int hydro_batch_set_tiling(hydro_batch* batch, int blocks_per_cu, int tile_rows, int tile_windows);
 int hydro_batch_get_tiling(hydro_batch* batch, int* blocks_per_cu, int* tile_rows, int* tile_windows);
 

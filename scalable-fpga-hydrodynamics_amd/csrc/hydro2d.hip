@@ -177,23 +177,6 @@ constexpr int default_occ_2d()
 // use shorter strips (down to 8 rows) until they have that many.
 constexpr long long kFillBlocks2 = 2048;
 
-// Dynamic LDS that leaves room for exactly `occ` workgroups of Kernel per CU (160 KiB of
-// LDS): the middle of the interval (L/(occ+1), L/occ], less the kernel's static LDS, so
-// allocation granularity cannot move the count.  Fewer waves in flight per CU keep fewer
-// DRAM rows open at once; the step kernels need no LDS of their own for this.
-template <auto Kernel>
-size_t occupancy_lds(int occ)
-{
-    if (occ <= 0) return 0;
-    static const size_t static_lds = [] {
-        hipFuncAttributes fa;
-        return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(Kernel)) == hipSuccess ? fa.sharedSizeBytes
-                                                                                              : size_t(0);
-    }();
-    const size_t total = (kLdsPerCu / (occ + 1) + kLdsPerCu / occ) / 2;
-    return total > static_lds ? total - static_lds : 0;
-}
-
 template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
                   T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,

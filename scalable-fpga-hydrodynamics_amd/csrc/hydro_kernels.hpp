@@ -35,6 +35,23 @@ int ic_blast_async_f64(double* d_rhoE, double* d_mom, int dim, const size_t* nb,
                        double p_in, double p_out, int ic_mode, double radius_frac, double C_min, double gamma,
                        unsigned long long* d_flags, double* dt_in, double* dt_out, hipStream_t s);
 
+// Dynamic LDS that leaves room for exactly `occ` workgroups of Kernel per CU (160 KiB of
+// LDS): the middle of the interval (L/(occ+1), L/occ], less the kernel's static LDS, so
+// allocation granularity cannot move the count.  Fewer waves in flight per CU keep fewer
+// DRAM rows open at once; the step kernels need no LDS of their own for this.
+template <auto Kernel>
+size_t occupancy_lds(int occ)
+{
+    if (occ <= 0) return 0;
+    static const size_t static_lds = [] {
+        hipFuncAttributes fa;
+        return hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(Kernel)) == hipSuccess ? fa.sharedSizeBytes
+                                                                                              : size_t(0);
+    }();
+    const size_t total = (kLdsPerCu / (occ + 1) + kLdsPerCu / occ) / 2;
+    return total > static_lds ? total - static_lds : 0;
+}
+
 // Launch geometry knobs (0 = per-precision default); get_*(true) returns the geometry of
 // the last launch after the strip adaptation.
 void set_rows_2d(int rows);

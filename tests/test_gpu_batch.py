@@ -103,17 +103,21 @@ def test_batch_add_orders_after_pending_buffer_work(gpu, oracle):
 
 def test_batch2d_tiling_changes_mid_run(gpu, oracle):
     batch, slots = _make(gpu, 2, SIZES_2D, 30)
-    for bpc, rows, width in [(1, 0, 1), (4, 32, 2), (2, 128, 4), (3, 0, 4), (2, 0, 2)]:
+    # Persistent (balanced, multi-window) and direct-dispatch (one-window R-row) tilings.
+    for bpc, rows, width in [(1, 0, 1), (0, 16, 1), (4, 32, 2), (3, 4, 1), (2, 128, 4), (4, 64, 1)]:
         batch.set_tiling(bpc, rows, width)
-        assert batch.tiling() == (bpc, rows, width)
-        batch.advance(6)
+        assert batch.tiling() == (bpc if bpc else 4, rows, width)
+        batch.advance(5)
     for slot, nb, C, rf, p_in, mi in slots:
         _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 30), f"2D tiling {nb}")
     batch.close()
 
 
-def test_batch2d_graph_replay(gpu, oracle):
+@pytest.mark.parametrize("tiling", [None, (3, 32, 1)])
+def test_batch2d_graph_replay(gpu, oracle, tiling):
     batch, slots = _make(gpu, 2, SIZES_2D[:6], 40)
+    if tiling:
+        batch.set_tiling(*tiling)
     batch.set_graph(4)
     batch.advance(10)  # two replays + two direct steps
     x = torch.randn(1 << 20, device="cuda")  # eager work between replays
@@ -132,8 +136,9 @@ def test_batch2d_autotune_steps_are_solve_steps(gpu, oracle):
     ms = batch.autotune(steps_per_trial=1, use_cache=False)
     assert ms > 0
     tiling = batch.tiling()
-    # blocks per CU 2..4 x rows (balanced, 32, 128) x width (1, 2, 4), then the best 3 again
-    ntrials = 3 * 3 * 3 + 3
+    # FP64: direct dispatch, workgroups per CU 3..4 x tile rows (16, 32, 64), and the
+    # balanced partition on 2..4 workgroups per CU; then the best 3 again
+    ntrials = 2 * 3 + 3 + 3
     batch.advance(40 - ntrials)
     for slot, nb, C, rf, p_in, mi in slots:
         _check_slot(batch, slot, nb, _oracle(oracle, nb, C, rf, p_in, 40), f"2D tuned {tiling} {nb}")
