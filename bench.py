@@ -276,7 +276,8 @@ def bench_batched(args, name: str) -> dict:
         "config": {"workload": name, "blasts": len(blasts), "cells": cells, "sides": "log-uniform [64, 2048]",
                    "tiling": {"blocks_per_cu": tuning[0], "tile_rows": tuning[1], "tile_windows": tuning[2]},
                    "graph_steps": graph_steps,
-                   "kernel": "batch2d (persistent, one launch per step) + batch finalize"},
+                   "kernel": "batch2d (one launch per step: direct dispatch of tiles or a persistent grid, "
+                             "as tuned) + batch finalize"},
         "roofline": {"bound": "hbm", "achieved": round(64 * cells / per_step / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(64 * cells / per_step / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": None,

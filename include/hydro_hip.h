@@ -186,8 +186,9 @@ int hydro_ic_f32(float* d_rhoE, float* d_mom, int dim, const size_t* nb, const s
  * of a batch is one complete run of hydro2d/main.cxx:686-815 (hydro3d/main.cxx:742-876)
  * with the reference's boundaries (X periodic, Y and Z reflective) and step arithmetic
  * (hydro2d/kernel.cxx:132-319, hydro3d/kernel.cxx:145-407), bit-identical to a
- * single-domain hydro_run of the same blast.  One persistent kernel per step advances
- * every active blast, followed by one finalise launch that computes each blast's Dt,
+ * single-domain hydro_run of the same blast.  One kernel launch per step advances
+ * every active blast (a persistent grid walking a segment table, or one wave per tile
+ * dispatched in table order; see hydro_batch_set_tiling), followed by one finalise launch that computes each blast's Dt,
  * advances its t / iteration count and flips its current buffer; a blast stops on its
  * own when !(t < max_t && it < max_it).  Steps may be replayed from a hipGraph.  The
  * batch runs on its own non-blocking stream (hydro_batch_stream): work queued on a

@@ -330,7 +330,7 @@ class Run:
 
 
 class Batch:
-    """Many independent blasts advanced together by one persistent kernel per step
+    """Many independent blasts advanced together by one kernel launch per step
     (include/hydro_hip.h, hydro_batch_*).  Each blast is a full run of the reference step
     loop with X periodic and Y/Z reflective boundaries; its buffers (a, b) are pairs
     (rhoE, mom) of contiguous float64 tensors in the reference layout."""
