@@ -67,10 +67,10 @@ struct BatchSeg {
 
 namespace {
 
-// Waves per SIMD of the 2D batch kernel (= workgroups per CU): the FP32-flux form fits
-// five in 96 VGPRs without spilling, the FP64 form four (up to 128 VGPRs).
+// Waves per SIMD of the 2D batch kernel (= workgroups per CU): four, at most 128 VGPRs
+// (FP64 uses 128, FP32-flux 96; FP32-flux at five would spill).
 template <typename TF>
-constexpr int kBatchWaves2 = sizeof(TF) == 4 ? 5 : 4;
+constexpr int kBatchWaves2 = 4;
 
 __device__ __forceinline__ void flush_key(BatchBlast* __restrict__ blasts, int b, uint64_t key, unsigned wid, int lane)
 {
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kBlock, kBatchWaves2<TF>)
     const int wv = threadIdx.x / kWave;
     const int g = __builtin_amdgcn_readfirstlane(wv % group);
     const unsigned wid = __builtin_amdgcn_readfirstlane(lb * (kWavesPerBlock / group) + wv / group);
-    if (wid >= nworkers) return;  // this kernel has no workgroup barrier
+    if (wid >= nworkers) return;  // never in direct mode (its only barrier is at the end)
     const TF gamma = static_cast<TF>(cs.gamma), gmo = static_cast<TF>(cs.gmo);
     const TF divgamma = static_cast<TF>(cs.divgamma), K = static_cast<TF>(cs.K);
     // Direct dispatch: wave `wid` takes segment `wid` of one table.
@@ -129,6 +129,31 @@ __global__ void __launch_bounds__(kBlock, kBatchWaves2<TF>)
         window_lane2d<TF, true>(j, nb_y, false, jl, ymask);
         key = march2d<TS, TF, M>(R2, M2, RN2, MN2, ys, nb_x, sg.i0, sg.i1, j, jl, ymask, owned, true, gamma, gmo,
                                  divgamma, K, DtDx, DtDy, key);
+    }
+    if (direct) {
+        // Every wave of a direct workgroup reaches this point.  Its waves hold consecutive
+        // tiles, mostly of one blast: fold their keys so that the workgroup issues one
+        // atomic per blast, as the plain step does (one per wave made 60% of the step on
+        // one large blast with 12-row tiles, serialised on the blast's 64 slots).
+        __shared__ unsigned long long s_key[kWavesPerBlock];
+        __shared__ int s_blast[kWavesPerBlock];
+        const uint64_t k = wave_max_u64(key);
+        if (lane == 0) {
+            s_key[wv] = k;
+            s_blast[wv] = cb;
+        }
+        __syncthreads();
+        if (threadIdx.x < kWavesPerBlock) {
+            const int t = threadIdx.x;
+            const int bl = s_blast[t];
+            if (bl >= 0 && (t == 0 || s_blast[t - 1] != bl)) {
+                uint64_t m = s_key[t];
+                for (int u = t + 1; u < kWavesPerBlock && s_blast[u] == bl; ++u) m = umax64(m, s_key[u]);
+                if (m != 0)
+                    atomicMax(&blasts[bl].ctl.red.slot[blockIdx.x % kRedSlots], static_cast<unsigned long long>(m));
+            }
+        }
+        return;
     }
     if (cb >= 0) flush_key(blasts, cb, key, wid, lane);
 }
