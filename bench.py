@@ -141,15 +141,54 @@ class EnergyMeter:
         except Exception as e:  # noqa: BLE001
             self.error = f"amdsmi unavailable: {e}"
 
-    def joules(self):
+    def sample(self):
+        """(counter timestamp in ns, accumulated joules), or None."""
         if self.h is None:
             return None
         try:
             e = self.smi.amdsmi_get_energy_count(self.h)
-            return e["energy_accumulator"] * e["counter_resolution"] * 1e-6
+            return int(e["timestamp"]), e["energy_accumulator"] * e["counter_resolution"] * 1e-6
         except Exception as e:  # noqa: BLE001
             self.error = f"energy counter: {e}"
             return None
+
+    def busy_power(self, busy):
+        """Average power while `busy()` keeps the device working (it enqueues work and
+        waits for it).  The accumulator advances about once a second, so the power is
+        taken between two of its own updates (joules over its own timestamps), using only
+        updates that fall inside the busy period; a host-timed window would catch a
+        variable number of updates and partly idle time.  Returns (watts, seconds
+        spanned) or None."""
+        import threading
+
+        samples = []
+        stop = threading.Event()
+
+        def poll():
+            last = None
+            while not stop.is_set():
+                cur = self.sample()
+                if cur is None:
+                    return
+                if last is None or cur[0] != last[0]:
+                    samples.append((time.perf_counter(), cur[0], cur[1]))
+                    last = cur
+                time.sleep(0.005)
+
+        th = threading.Thread(target=poll, daemon=True)
+        th.start()
+        h0 = time.perf_counter()
+        busy()
+        h1 = time.perf_counter()
+        stop.set()
+        th.join()
+        inside = [x for x in samples if h0 < x[0] <= h1]
+        # The first update inside may close an interval that began before the work did.
+        if len(inside) < 3:
+            self.error = self.error or f"energy counter advanced {len(inside)} times in {h1 - h0:.1f} s of work"
+            return None
+        (_, t0, j0), (_, t1, j1) = inside[1], inside[-1]
+        return (j1 - j0) / ((t1 - t0) * 1e-9), (t1 - t0) * 1e-9
 
 
 def load_traffic(workload: str):
@@ -490,7 +529,8 @@ def main() -> int:
     log(f"[bench] IC done: dt0={dt0:.3e} mass0={mass0:.7e}")
 
     W, K = args.warmup, args.steps
-    run = hydro.Run(nb, k["D"], k["C_min"], dt0, 2.0, W + K, precision)
+    E_MAX = 4000  # cap on the energy window's steps (see below)
+    run = hydro.Run(nb, k["D"], k["C_min"], dt0, 2.0, W + K + E_MAX, precision)
     run.advance(a, b, W)
     st = run.status()
     if st["rc"] != 0:
@@ -502,14 +542,12 @@ def main() -> int:
     meter = EnergyMeter()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    j0 = meter.joules()
     w0 = time.perf_counter()
     ev0.record()
     run.advance(a, b, K)
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
-    j1 = meter.joules()
     gpu_s = ev0.elapsed_time(ev1) / 1e3
     kernel_ms, launches = run.kernel_time()
     st = run.status()
@@ -520,6 +558,24 @@ def main() -> int:
     mass1 = hydro.total_mass(cur[0])
     cells = int(np.prod(nb))
     elapsed = max(wall, gpu_s)
+
+    # Energy: average power between updates of the accumulator (about one a second)
+    # while the same resident steps keep the device busy (about 5.5 s of them enqueued),
+    # times the timed step time.
+    run.set_timing(False)
+    E = min(E_MAX, max(K, int(np.ceil(5.5 / (elapsed / K)))))
+    torch.cuda.synchronize()
+
+    def busy():
+        run.advance(a, b, E)
+        torch.cuda.synchronize()
+
+    bp = meter.busy_power(busy)
+    st_e = run.status()
+    if st_e["rc"] != 0 or st_e["it"] != W + K + E:
+        log(f"[bench] energy window failed: {st_e}")
+        return 1
+    cur = a if st_e["current"] == 0 else b
     gcups = cells * K / elapsed / 1e9
     avg_kernel_s = kernel_ms / 1e3 / max(launches, 1)
     achieved = bpc * cells / avg_kernel_s / 1e9
@@ -606,9 +662,11 @@ def main() -> int:
         "achievable_write_gbs": round(write_gbs, 1) if write_gbs else None,
         "frac_of_achievable": round(achieved / copy_gbs, 4) if copy_gbs else None,
         "kernel_ms": round(avg_kernel_s * 1e3, 4),
-        "energy": ({"joules_per_step": round((j1 - j0) / K, 4), "nj_per_cell_update": round((j1 - j0) / K / cells * 1e9, 4),
-                    "avg_watts": round((j1 - j0) / wall, 1), "source": "amdsmi energy accumulator over the timed steps"}
-                   if j0 is not None and j1 is not None and j1 > j0 else {"error": meter.error or "no energy delta"}),
+        "energy": ({"avg_watts": round(bp[0], 1), "joules_per_step": round(bp[0] * elapsed / K, 4),
+                    "nj_per_cell_update": round(bp[0] * elapsed / K / cells * 1e9, 4), "window_s": round(bp[1], 3),
+                    "source": "amdsmi energy accumulator: average power between its own updates while the same "
+                              "resident steps run, times the timed step time"}
+                   if bp is not None else {"error": meter.error or "no energy reading"}),
         "mass_change_per_cell": abs(mass1 - mass0) / cells,
         "t_final": st["t"],
     }
