@@ -208,6 +208,45 @@ def _blast_bufs(torch, nb):
     return a, [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
 
 
+def _prims2(torch, rhoE, mom):
+    """rho, p, u, v of the interior cells (gamma = 1.4)."""
+    rho = rhoE[1:-1, 1:-1, 0]
+    E = rhoE[1:-1, 1:-1, 1]
+    u = mom[1:-1, 1:-1, 0] / rho
+    v = mom[1:-1, 1:-1, 1] / rho
+    p = 0.4 * (E - 0.5 * rho * (u * u + v * v))
+    return {"rho": rho, "p": p, "u": u, "v": v}
+
+
+def _batch_error_vs_fp64(hydro, torch, blasts, batch, steps):
+    """Worst relative L-inf per field over all blasts of an FP32-flux batch against an
+    FP64 batch of the same blasts after the same number of steps: rho and p relative to
+    their own maximum, velocities relative to the FP64 solve's maximum signal speed."""
+    ref_bufs = [_blast_bufs(torch, p["nb"]) for p in blasts]
+    torch.cuda.synchronize()
+    ref = hydro.Batch(2, hydro.FP64, len(blasts))
+    for p, (a, b) in zip(blasts, ref_bufs):
+        ref.add_blast(a, b, p["C"], p["radius_frac"], p["p_in"], p["p_out"], p["rho_in"], p["rho_out"],
+                      max_t=1e30, max_it=10**9)
+    ref.advance(steps)
+    ref.sync()
+    worst = {"rho": 0.0, "p": 0.0, "u": 0.0, "v": 0.0}
+    for slot in range(len(blasts)):
+        if ref.status(slot)["it"] != steps or batch.status(slot)["it"] != steps:
+            raise RuntimeError(f"FP64 reference of blast {slot} is not at step {steps}")
+        g = _prims2(torch, *batch.current(slot))
+        w = _prims2(torch, *ref.current(slot))
+        vscale = float(torch.max(torch.sqrt(1.4 * w["p"].abs() / w["rho"]) + torch.sqrt(w["u"] ** 2 + w["v"] ** 2)))
+        for k in worst:
+            scale = vscale if k in ("u", "v") else float(w[k].abs().max())
+            worst[k] = max(worst[k], float((g[k] - w[k]).abs().max()) / scale)
+    ref.close()
+    del ref_bufs
+    return {**{k: float(f"{v:.3e}") for k, v in worst.items()}, "steps": int(steps), "bound": 1e-5,
+            "metric": "worst relative L-inf over the blasts: rho and p against their own max, u and v against "
+                      "the FP64 solve's max signal speed"}
+
+
 def bench_batched(args, name: str) -> dict:
     """256 independent 2D blasts of uneven sizes in one batch: autotuned tiling, steps
     replayed from a hipGraph, K timed steps.  Beside it, the same blasts solved one at a
@@ -259,6 +298,11 @@ def bench_batched(args, name: str) -> dict:
     if any(flags) or bad:
         raise RuntimeError(f"batched run: unexpected stop flags / status {flags.nonzero()} {bad}")
     log(f"[bench] batched: {K} steps in {elapsed * 1e3:.1f} ms, {gcups:.2f} GCUPS")
+    err = None
+    if precision == hydro.FP32_FLUX:
+        err = _batch_error_vs_fp64(hydro, torch, blasts, batch, st[0]["it"])
+        log(f"[bench] FP32-flux vs FP64 after {err['steps']} steps, worst rel L-inf: "
+            + ", ".join(f"{k2} {err[k2]:.2e}" for k2 in ("rho", "p", "u", "v")))
     batch.close()
 
     # One solve at a time, same blasts, a sample of steps each (GCUPS is per-step).
@@ -325,6 +369,7 @@ def bench_batched(args, name: str) -> dict:
         "speedup_vs_plain_resident": round(gcups / plain_res, 3),
         "speedup_vs_plain_sync_launcher": round(gcups / plain_sync, 3) if plain_sync else None,
         "plain_sample": f"{ks} steps per blast, all {len(blasts)} blasts (resident) / first {n_sync} (synchronous)",
+        "error_vs_fp64": err,
     }
 
 
@@ -683,7 +728,7 @@ def main() -> int:
         line["forms"] = {
             "batched2d_256_fp32flux": {k: bt[k] for k in (
                 "value", "unit", "ms_per_step", "plain_resident_gcups", "plain_sync_launcher_gcups",
-                "speedup_vs_plain_resident", "speedup_vs_plain_sync_launcher", "plain_sample")}
+                "speedup_vs_plain_resident", "speedup_vs_plain_sync_launcher", "plain_sample", "error_vs_fp64")}
             | {"tiling": bt["config"]["tiling"], "roofline_achieved_gbs": bt["roofline"]["achieved"]},
             "served_mixed": {"requests": sv["config"]["requests"], "arrival_rate_per_s": sv["config"]["arrival_rate_per_s"],
                              "load": sv["config"]["load"], "served": sv["served"],
