@@ -362,7 +362,8 @@ def bench_batched(args, name: str) -> dict:
                    "kernel": "batch2d (one launch per step: direct dispatch of tiles or a persistent grid, "
                              "as tuned) + batch finalize"},
         "roofline": {"bound": "hbm", "achieved": round(64 * cells / per_step / 1e9, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(64 * cells / per_step / 1e9 / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(64 * cells / per_step / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": load_traffic(name)},
         "cpu_baseline": None,
         "plain_resident_gcups": round(plain_res, 4),
         "plain_sync_launcher_gcups": round(plain_sync, 4) if plain_sync else None,
