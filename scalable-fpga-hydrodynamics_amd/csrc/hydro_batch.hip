@@ -278,7 +278,6 @@ struct hydro_batch {
     int blocks_per_cu = 0;  // 0: default
     int tile_rows = 0;      // 0: balanced
     int group = 1;          // 2D: adjacent windows per worker (1, 2 or 4)
-    int eff_rows = 0;       // tile rows after the direct-dispatch adaptation
     unsigned grid = 0, nworkers = 0;
     bool tuned = false;
     // Graph of `graph_steps` steps.
@@ -352,7 +351,6 @@ int build_tables(hydro_batch* b)
         for (int c = 0; c < ncol; ++c) cols.push_back({ static_cast<int>(s), c, b->h_blasts[s].nb_x });
     }
     if (b->tile_rows <= 0) {
-        b->eff_rows = 0;
         long long total = 0;
         for (const Col& c : cols) total += c.rows;
         // Worker w takes rows [w*total/W, (w+1)*total/W) of the concatenation.
@@ -392,7 +390,6 @@ int build_tables(hydro_batch* b)
             };
             while (R > 8 && (tiles(R) + kWavesPerBlock - 1) / kWavesPerBlock < kDirectFill) R = std::max(8, R / 2);
         }
-        b->eff_rows = R;
         size_t t = 0;
         for (size_t c0 = 0; c0 < cols.size();) {
             size_t c1 = c0;
