@@ -624,7 +624,7 @@ std::string signature(const hydro_batch* b)
 }  // namespace
 
 extern "C" hydro_batch* hydro_batch_create(int dim, int precision, size_t capacity)
-{
+try {
     clear_error();
     if ((dim != 2 && dim != 3) || (precision != HYDRO_FP64 && precision != HYDRO_FP32_FLUX) || capacity < 1 ||
         capacity > (1u << 20)) {
@@ -663,10 +663,13 @@ extern "C" hydro_batch* hydro_batch_create(int dim, int precision, size_t capaci
         return nullptr;
     }
     return b;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return nullptr;
 }
 
 extern "C" void hydro_batch_destroy(hydro_batch* b)
-{
+try {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     drop_graph(b);
@@ -676,13 +679,15 @@ extern "C" void hydro_batch_destroy(hydro_batch* b)
     if (b->d_off) (void)hipFree(b->d_off);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+} catch (...) {
+    hydro::set_exception_error(__func__);
 }
 
 extern "C" void* hydro_batch_stream(hydro_batch* b) { return b ? static_cast<void*>(b->stream) : nullptr; }
 
 extern "C" int hydro_batch_add(hydro_batch* b, const size_t* nb, const double* D, double C_min, double dt0,
                                double max_t, size_t max_it, void* rhoE_a, void* mom_a, void* rhoE_b, void* mom_b)
-{
+try {
     if (!check(b, "hydro_batch_add")) return HYDRO_ERR_ARG;
     const int slot = add_common(b, nb, D, C_min, max_t, max_it, rhoE_a, mom_a, rhoE_b, mom_b, "hydro_batch_add");
     if (slot < 0) return slot;
@@ -693,12 +698,15 @@ extern "C" int hydro_batch_add(hydro_batch* b, const size_t* nb, const double* D
         return rc;
     }
     return slot;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_add_blast(hydro_batch* b, const size_t* nb, double C, double radius_frac, double rho_in,
                                      double rho_out, double p_in, double p_out, double max_t, size_t max_it,
                                      void* rhoE_a, void* mom_a, void* rhoE_b, void* mom_b)
-{
+try {
     if (!check(b, "hydro_batch_add_blast")) return HYDRO_ERR_ARG;
     if (!nb) {
         set_error("hydro_batch_add_blast: null nb");
@@ -752,10 +760,13 @@ extern "C" int hydro_batch_add_blast(hydro_batch* b, const size_t* nb, double C,
         return rc;
     }
     return slot;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_remove(hydro_batch* b, int slot)
-{
+try {
     if (!check(b, "hydro_batch_remove")) return HYDRO_ERR_ARG;
     if (slot < 0 || static_cast<size_t>(slot) >= b->cap || !b->used[slot]) {
         set_error("hydro_batch_remove: slot %d is not in use", slot);
@@ -771,24 +782,33 @@ extern "C" int hydro_batch_remove(hydro_batch* b, int slot)
     if (rc) return rc;
     HYDRO_HIP_OR(hipStreamSynchronize(b->stream), HYDRO_ERR_HIP);
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_advance(hydro_batch* b, size_t n_steps)
-{
+try {
     if (!check(b, "hydro_batch_advance")) return HYDRO_ERR_ARG;
     return advance_impl(b, n_steps);
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_sync(hydro_batch* b)
-{
+try {
     if (!check(b, "hydro_batch_sync")) return HYDRO_ERR_ARG;
     HYDRO_HIP_OR(hipStreamSynchronize(b->stream), HYDRO_ERR_HIP);
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_status(hydro_batch* b, int slot, size_t* it, double* t, double* dt, int* done,
                                   int* current)
-{
+try {
     if (!check(b, "hydro_batch_status")) return HYDRO_ERR_ARG;
     if (slot < 0 || static_cast<size_t>(slot) >= b->cap || !b->used[slot]) {
         set_error("hydro_batch_status: slot %d is not in use", slot);
@@ -807,10 +827,13 @@ extern "C" int hydro_batch_status(hydro_batch* b, int slot, size_t* it, double* 
         return HYDRO_ERR_NONFINITE;
     }
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_done_flags(hydro_batch* b, unsigned char* flags, size_t n)
-{
+try {
     if (!check(b, "hydro_batch_done_flags")) return HYDRO_ERR_ARG;
     if (!flags || n > b->cap) {
         set_error("hydro_batch_done_flags: null buffer or n > capacity");
@@ -819,10 +842,13 @@ extern "C" int hydro_batch_done_flags(hydro_batch* b, unsigned char* flags, size
     HYDRO_HIP_OR(hipMemcpyAsync(flags, b->d_done, n, hipMemcpyDeviceToHost, b->stream), HYDRO_ERR_HIP);
     HYDRO_HIP_OR(hipStreamSynchronize(b->stream), HYDRO_ERR_HIP);
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_set_tiling(hydro_batch* b, int blocks_per_cu, int tile_rows, int tile_windows)
-{
+try {
     if (!check(b, "hydro_batch_set_tiling")) return HYDRO_ERR_ARG;
     const bool width_ok = tile_windows == 0 || tile_windows == 1 || (b->dim == 2 && (tile_windows == 2 || tile_windows == 4));
     if (blocks_per_cu < 0 || blocks_per_cu > max_blocks(b) || tile_rows < 0 || tile_rows > (1 << 30) || !width_ok) {
@@ -840,19 +866,25 @@ extern "C" int hydro_batch_set_tiling(hydro_batch* b, int blocks_per_cu, int til
     if (direct_mode(b) != was_direct) drop_graph(b);  // the grid and arguments differ
     b->dirty = true;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_get_tiling(hydro_batch* b, int* blocks_per_cu, int* tile_rows, int* tile_windows)
-{
+try {
     if (!check(b, "hydro_batch_get_tiling")) return HYDRO_ERR_ARG;
     if (blocks_per_cu) *blocks_per_cu = b->blocks_per_cu > 0 ? b->blocks_per_cu : default_blocks(b);
     if (tile_rows) *tile_rows = b->tile_rows;
     if (tile_windows) *tile_windows = b->group;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_set_graph(hydro_batch* b, size_t steps_per_graph)
-{
+try {
     if (!check(b, "hydro_batch_set_graph")) return HYDRO_ERR_ARG;
     if (steps_per_graph > 4096) {
         set_error("hydro_batch_set_graph: at most 4096 steps per graph");
@@ -861,10 +893,13 @@ extern "C" int hydro_batch_set_graph(hydro_batch* b, size_t steps_per_graph)
     if (steps_per_graph != b->graph_steps) drop_graph(b);
     b->graph_steps = steps_per_graph;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_batch_autotune(hydro_batch* b, size_t steps_per_trial, int use_cache, double* best_ms_per_step)
-{
+try {
     if (!check(b, "hydro_batch_autotune")) return HYDRO_ERR_ARG;
     if (steps_per_trial < 1) {
         set_error("hydro_batch_autotune: steps_per_trial must be >= 1");
@@ -944,4 +979,7 @@ extern "C" int hydro_batch_autotune(hydro_batch* b, size_t steps_per_trial, int 
     }
     if (best_ms_per_step) *best_ms_per_step = best;
     return hydro_batch_set_tiling(b, best_c.bpc, best_c.rows, best_c.group);
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }

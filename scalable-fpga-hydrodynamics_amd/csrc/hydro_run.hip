@@ -69,7 +69,7 @@ void physics_constants(int precision, double C_min, double out[5])
 
 extern "C" hydro_run* hydro_run_create(int dim, const size_t* nb, const double* D, double C_min, double dt0,
                                        double max_t, size_t max_it, int precision, const int* bc)
-{
+try {
     clear_error();
     if ((dim != 2 && dim != 3) || !nb || !D ||
         (precision != HYDRO_FP64 && precision != HYDRO_FP32 && precision != HYDRO_FP32_FLUX)) {
@@ -122,11 +122,14 @@ extern "C" hydro_run* hydro_run_create(int dim, const size_t* nb, const double* 
         return nullptr;
     }
     return r;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return nullptr;
 }
 
 extern "C" int hydro_run_advance(hydro_run* r, void* rhoE_a, void* mom_a, void* rhoE_b, void* mom_b, size_t n_steps,
                                  void* stream)
-{
+try {
     clear_error();
     if (!r || !rhoE_a || !mom_a || !rhoE_b || !mom_b) {
         set_error("hydro_run_advance: null argument");
@@ -177,11 +180,14 @@ extern "C" int hydro_run_advance(hydro_run* r, void* rhoE_a, void* mom_a, void* 
         ++r->enqueued;
     }
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_run_status(hydro_run* r, size_t* it, double* t, double* dt, int* done, int* current,
                                 void* stream)
-{
+try {
     clear_error();
     if (!r) {
         set_error("hydro_run_status: null run");
@@ -202,11 +208,14 @@ extern "C" int hydro_run_status(hydro_run* r, size_t* it, double* t, double* dt,
         return HYDRO_ERR_NONFINITE;
     }
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_run_trajectory(hydro_run* r, size_t from, size_t count, double* t_out, double* dt_out,
                                     void* stream)
-{
+try {
     clear_error();
     if (!r || from + count > r->max_it) {
         set_error("hydro_run_trajectory: range [%zu, %zu) outside the %zu recorded iterations", from, from + count,
@@ -243,10 +252,13 @@ extern "C" int hydro_run_trajectory(hydro_run* r, size_t from, size_t count, dou
     }
     HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_run_set_timing(hydro_run* r, int enable)
-{
+try {
     clear_error();
     if (!r) {
         set_error("hydro_run_set_timing: null run");
@@ -255,10 +267,13 @@ extern "C" int hydro_run_set_timing(hydro_run* r, int enable)
     r->timing = enable != 0;
     r->ev_used = 0;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_run_kernel_time(hydro_run* r, double* total_ms, size_t* launches)
-{
+try {
     clear_error();
     if (!r || !total_ms) {
         set_error("hydro_run_kernel_time: null argument");
@@ -275,16 +290,21 @@ extern "C" int hydro_run_kernel_time(hydro_run* r, double* total_ms, size_t* lau
     if (launches) *launches = r->ev_used;
     r->ev_used = 0;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" void hydro_run_destroy(hydro_run* r)
-{
+try {
     if (!r) return;
     for (hipEvent_t e : r->ev) (void)hipEventDestroy(e);
     if (r->d_ctl) (void)hipFree(r->d_ctl);
     if (r->d_traj_t) (void)hipFree(r->d_traj_t);
     if (r->d_traj_dt) (void)hipFree(r->d_traj_dt);
     delete r;
+} catch (...) {
+    hydro::set_exception_error(__func__);
 }
 
 // ---------------------------------------------------------------------------------
@@ -416,12 +436,18 @@ int total_component(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, i
 
 extern "C" int hydro_total_mass(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision,
                                 double* mass, void* stream)
-{
+try {
     return total_component(d_rhoE, NB_X, NB_Y, NB_Z, precision, 0, mass, stream, "hydro_total_mass");
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_total_energy(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, int precision,
                                   double* energy, void* stream)
-{
+try {
     return total_component(d_rhoE, NB_X, NB_Y, NB_Z, precision, 1, energy, stream, "hydro_total_energy");
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }

@@ -6,6 +6,9 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <exception>
+#include <new>
+
 #include "../../include/hydro_hip.h"
 
 namespace {
@@ -23,6 +26,19 @@ void set_error(const char* fmt, ...)
 }
 
 void clear_error() { g_last_error[0] = '\0'; }
+
+void set_exception_error(const char* fn)
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc&) {
+        set_error("%s: out of host memory", fn);
+    } catch (const std::exception& e) {
+        set_error("%s: %s", fn, e.what());
+    } catch (...) {
+        set_error("%s: unexpected exception", fn);
+    }
+}
 
 int current_device()
 {

@@ -11,6 +11,10 @@ namespace hydro {
 // Record a printf-style message as this thread's last error.
 void set_error(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 void clear_error();
+// Record the exception being handled (host allocation failures in the batch, server and
+// run APIs) as this thread's last error; for the catch clauses that keep C++
+// exceptions from crossing the C ABI.
+void set_exception_error(const char* fn);
 
 // Current device index, or -1 with the error recorded.
 int current_device();

@@ -264,7 +264,7 @@ int retire(hydro_server* s, int e, size_t* n_done)
 }  // namespace
 
 extern "C" hydro_server* hydro_server_create(size_t hbm_bytes, size_t capacity, size_t steps_per_tick)
-{
+try {
     clear_error();
     if (hbm_bytes < kAlign || capacity < 1 || capacity > (1u << 16) || steps_per_tick < 1) {
         set_error("hydro_server_create: hbm_bytes >= 256, capacity in [1, 65536], steps_per_tick >= 1");
@@ -287,10 +287,13 @@ extern "C" hydro_server* hydro_server_create(size_t hbm_bytes, size_t capacity, 
     }
     s->arena.reset(hbm_bytes);
     return s;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return nullptr;
 }
 
 extern "C" void hydro_server_destroy(hydro_server* s)
-{
+try {
     if (!s) return;
     for (int e = 0; e < 4; ++e) {
         if (s->eng[e]) hydro_batch_destroy(s->eng[e]);
@@ -298,6 +301,8 @@ extern "C" void hydro_server_destroy(hydro_server* s)
     }
     if (s->base) (void)hipFree(s->base);
     delete s;
+} catch (...) {
+    hydro::set_exception_error(__func__);
 }
 
 extern "C" double hydro_server_now(hydro_server* s) { return s ? s->now() : 0.0; }
@@ -305,7 +310,7 @@ extern "C" double hydro_server_now(hydro_server* s) { return s ? s->now() : 0.0;
 extern "C" int hydro_server_submit(hydro_server* s, int dim, const size_t* nb, int precision, double C,
                                    double radius_frac, double rho_in, double rho_out, double p_in, double p_out,
                                    double max_t, size_t max_it, int keep_result, double arrival)
-{
+try {
     if (!check(s, "hydro_server_submit")) return HYDRO_ERR_ARG;
     if ((dim != 2 && dim != 3) || !nb || (precision != HYDRO_FP64 && precision != HYDRO_FP32_FLUX)) {
         set_error("hydro_server_submit: dim 2 or 3, nb non-null, precision HYDRO_FP64 or HYDRO_FP32_FLUX");
@@ -343,14 +348,24 @@ extern "C" int hydro_server_submit(hydro_server* s, int dim, const size_t* nb, i
                   s->arena.size());
         return HYDRO_ERR_NOMEM;
     }
-    s->reqs.push_back(r);
-    const int id = static_cast<int>(s->reqs.size() - 1);
+    // Queue first: if the request list cannot grow, the queue entry is taken back, so a
+    // failed submit leaves neither a queued id without a request nor the reverse.
+    const int id = static_cast<int>(s->reqs.size());
     s->queue.push_back(id);
+    try {
+        s->reqs.push_back(r);
+    } catch (...) {
+        s->queue.pop_back();
+        throw;
+    }
     return id;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_tick(hydro_server* s, size_t* n_done)
-{
+try {
     if (!check(s, "hydro_server_tick")) return HYDRO_ERR_ARG;
     if (n_done) *n_done = 0;
     // Wait until an engine that has work (running requests, or queued ones it could
@@ -394,20 +409,26 @@ extern "C" int hydro_server_tick(hydro_server* s, size_t* n_done)
         }
     ++s->ticks;
     return rc;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_pending(hydro_server* s, size_t* queued, size_t* running, size_t* bytes_in_use)
-{
+try {
     if (!check(s, "hydro_server_pending")) return HYDRO_ERR_ARG;
     if (queued) *queued = s->queue.size();
     if (running) *running = s->running[0] + s->running[1] + s->running[2] + s->running[3];
     if (bytes_in_use) *bytes_in_use = s->arena.used();
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_request(hydro_server* s, int id, int* state, double* t_arrival, double* t_admit,
                                     double* t_done, size_t* it, double* t_sim)
-{
+try {
     if (!check(s, "hydro_server_request")) return HYDRO_ERR_ARG;
     if (id < 0 || static_cast<size_t>(id) >= s->reqs.size()) {
         set_error("hydro_server_request: unknown request %d", id);
@@ -421,10 +442,13 @@ extern "C" int hydro_server_request(hydro_server* s, int id, int* state, double*
     if (it) *it = r.it;
     if (t_sim) *t_sim = r.t_sim;
     return r.rc;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_result(hydro_server* s, int id, void* dst_rhoE, void* dst_mom)
-{
+try {
     if (!check(s, "hydro_server_result")) return HYDRO_ERR_ARG;
     if (id < 0 || static_cast<size_t>(id) >= s->reqs.size() || s->reqs[id].state != kDone) {
         set_error("hydro_server_result: request %d has no kept result", id);
@@ -439,10 +463,13 @@ extern "C" int hydro_server_result(hydro_server* s, int id, void* dst_rhoE, void
     if (dst_mom)
         HYDRO_HIP_OR(hipMemcpy(dst_mom, mom, cells * static_cast<size_t>(r.dim) * 8, hipMemcpyDefault), HYDRO_ERR_HIP);
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_release(hydro_server* s, int id)
-{
+try {
     if (!check(s, "hydro_server_release")) return HYDRO_ERR_ARG;
     if (id < 0 || static_cast<size_t>(id) >= s->reqs.size() || s->reqs[id].state != kDone) {
         set_error("hydro_server_release: request %d has no kept result", id);
@@ -450,10 +477,13 @@ extern "C" int hydro_server_release(hydro_server* s, int id)
     }
     release_memory(s, s->reqs[id]);
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_set_limits(hydro_server* s, double max_active_cells, size_t steps_per_tick)
-{
+try {
     if (!check(s, "hydro_server_set_limits")) return HYDRO_ERR_ARG;
     if (!(max_active_cells >= 0) || steps_per_tick < 1) {
         set_error("hydro_server_set_limits: max_active_cells >= 0, steps_per_tick >= 1");
@@ -462,12 +492,18 @@ extern "C" int hydro_server_set_limits(hydro_server* s, double max_active_cells,
     s->max_active_cells = max_active_cells;
     s->steps_per_tick = steps_per_tick;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
 
 extern "C" int hydro_server_stats(hydro_server* s, size_t* ticks, double* cell_updates)
-{
+try {
     if (!check(s, "hydro_server_stats")) return HYDRO_ERR_ARG;
     if (ticks) *ticks = s->ticks;
     if (cell_updates) *cell_updates = s->cell_updates;
     return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
 }
