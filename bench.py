@@ -502,6 +502,9 @@ def main() -> int:
     ap.add_argument("--occ", type=int, default=0, help="override the 2D kernel's workgroups per CU (3..8)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-reference", action="store_true", help="skip the achievable-bandwidth copy")
+    ap.add_argument("--no-energy", action="store_true",
+                    help="skip the energy window (about 5.5 s of further steps at the power limit), e.g. "
+                         "under a profiler, so that every traced step launch is a timed one")
     ap.add_argument("--no-forms", action="store_true",
                     help="default workload only: skip the batched and served summaries")
     ap.add_argument("--plain-steps", type=int, default=20, help="steps per blast of the one-solve-at-a-time baselines")
@@ -609,19 +612,23 @@ def main() -> int:
     # while the same resident steps keep the device busy (about 5.5 s of them enqueued),
     # times the timed step time.
     run.set_timing(False)
-    E = min(E_MAX, max(K, int(np.ceil(5.5 / (elapsed / K)))))
-    torch.cuda.synchronize()
-
-    def busy():
-        run.advance(a, b, E)
+    bp = None
+    if args.no_energy:
+        meter.error = "not measured (--no-energy)"
+    else:
+        E = min(E_MAX, max(K, int(np.ceil(5.5 / (elapsed / K)))))
         torch.cuda.synchronize()
 
-    bp = meter.busy_power(busy)
-    st_e = run.status()
-    if st_e["rc"] != 0 or st_e["it"] != W + K + E:
-        log(f"[bench] energy window failed: {st_e}")
-        return 1
-    cur = a if st_e["current"] == 0 else b
+        def busy():
+            run.advance(a, b, E)
+            torch.cuda.synchronize()
+
+        bp = meter.busy_power(busy)
+        st_e = run.status()
+        if st_e["rc"] != 0 or st_e["it"] != W + K + E:
+            log(f"[bench] energy window failed: {st_e}")
+            return 1
+        cur = a if st_e["current"] == 0 else b
     gcups = cells * K / elapsed / 1e9
     avg_kernel_s = kernel_ms / 1e3 / max(launches, 1)
     achieved = bpc * cells / avg_kernel_s / 1e9
@@ -633,7 +640,7 @@ def main() -> int:
     # second buffer pair as the destination after the run.
     copy_gbs = read_gbs = write_gbs = None
     if not args.no_copy_reference:
-        src, dst = cur[0], (b if st["current"] == 0 else a)[0]
+        src, dst = cur[0], (b if cur is a else a)[0]
         hydro.stream_copy(dst, src)
         torch.cuda.synchronize()
         c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

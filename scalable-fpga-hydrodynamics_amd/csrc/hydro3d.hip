@@ -4,12 +4,14 @@
 // `launcher` (hydro3d/kernel.cxx:419-455), called from hydro3d/main.cxx:791-793.
 //
 // Work decomposition:
-//   * a workgroup covers a (Y x Z) tile of TY rows x 62 columns and marches along X
+//   * a workgroup covers a (Y x Z) tile of TY rows x wz <= 62 columns (z_window_width:
+//     the columns of a row spread evenly over the fewest windows) and marches along X
 //     over a strip of rows; it has TY + 2 waves, wave w holding Y row j0 - 1 + w
 //     (waves 0 and TY+1 are halo rows: they load and convert their row but update
 //     nothing);
-//   * within a wave, lane l holds Z column k = 62*kw + l (lanes 0 and 63 are halo
-//     columns); the Z face (k, k+1) is computed once by lane k and shuffled to lane k+1;
+//   * within a wave, lane l holds Z column k = wz*kw + l (lanes 0 and wz+1 are halo
+//     columns, lanes above wz+1 idle); the Z face (k, k+1) is computed once by lane k
+//     and shuffled to lane k+1;
 //   * the Y face (j, j+1) is computed once by the wave of row j from the primitives
 //     the wave of row j+1 left in LDS, and left in LDS for the wave of row j+1
 //     (two workgroup barriers per X step, single-buffered LDS);
@@ -40,6 +42,7 @@ struct Geom3 {
     long long zs, zy;  // nb_z + 2, (nb_y + 2) * (nb_z + 2)
     int rows;          // X rows per strip
     int nzw, nyw;      // Z windows, Y tiles
+    int wz;            // owned Z columns per window (z_window_width)
     int wrap_x, wrap_y, wrap_z;  // periodic axes read from the opposite interior plane
 };
 
@@ -78,9 +81,9 @@ __global__ void __launch_bounds__(kBlock3, HYDRO_3D_MIN_WAVES)
     const int zw = static_cast<int>(rem % static_cast<unsigned>(g.nzw));
 
     const int j = yt * kTY + w;  // row of this wave (j0 - 1 + w with j0 = 1 + yt*TY)
-    const int k = zw * kOwnedZ + lane;
+    const int k = zw * g.wz + lane;
     const bool halo_wave = (w == 0) || (w == kNW - 1);
-    const bool owned = !halo_wave && j <= g.nb_y && lane >= 1 && lane <= kOwnedZ && k <= g.nb_z;
+    const bool owned = !halo_wave && j <= g.nb_y && lane >= 1 && lane <= g.wz && k <= g.nb_z;
     // Reflective walls as in 2D: a ghost row (wave) / ghost column (lane) loads its
     // interior neighbour and presents its mirror, normal velocity negated, to the face
     // (hydro3d/kernel.cxx:248-252, 269-273, 290-294, 311-315).
@@ -212,7 +215,8 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     g.nb_z = static_cast<int>(nb_z);
     g.zs = static_cast<long long>(nb_z) + 2;
     g.zy = (static_cast<long long>(nb_y) + 2) * g.zs;
-    g.nzw = static_cast<int>((nb_z + kOwnedZ - 1) / kOwnedZ);
+    g.nzw = z_windows(g.nb_z);
+    g.wz = z_window_width(g.nb_z);
     g.nyw = static_cast<int>((nb_y + kTY - 1) / kTY);
     g.wrap_x = (wraps & kWrapX) ? 1 : 0;
     g.wrap_y = (wraps & kWrapY) ? 1 : 0;

@@ -10,8 +10,9 @@
 // blasts finish independently.
 //
 // Work decomposition.  The interior of every blast is cut into "columns": a 2D column is
-// a 62-wide Y window over all X rows (one wave marches it), a 3D column a 6 x 62 (Y x Z)
-// tile over all X rows (one 8-wave workgroup marches it).  The persistent grid has a
+// a 62-wide Y window over all X rows (one wave marches it), a 3D column a 6-row Y tile of
+// a Z window over all X rows (one 8-wave workgroup marches it; the Z windows of a row
+// share its columns evenly, at most 62 each).  The persistent grid has a
 // fixed number of workers (waves in 2D, workgroups in 3D); a host-built segment table
 // gives each worker a list of (blast, column, rows i0..i1) segments.  Two tilings:
 //   balanced   the concatenated rows of all columns are split into equal contiguous
@@ -191,11 +192,11 @@ __global__ void __launch_bounds__(kBlock3, 4)
         const TS DtDy = dt / static_cast<TS>(B->ctl.D[1]);
         const TS DtDz = dt / static_cast<TS>(B->ctl.D[2]);
         const int nb_x = B->nb_x, nb_y = B->nb_y, nb_z = B->nb_z;
-        const int nzw = (nb_z + kOwnedZ - 1) / kOwnedZ;
+        const int nzw = z_windows(nb_z), wz = z_window_width(nb_z);
         const int yt = sg.col / nzw, zw = sg.col % nzw;
         const int j = yt * kTY + w;
-        const int k = zw * kOwnedZ + lane;
-        const bool owned = !halo_wave && j <= nb_y && lane >= 1 && lane <= kOwnedZ && k <= nb_z;
+        const int k = zw * wz + lane;
+        const bool owned = !halo_wave && j <= nb_y && lane >= 1 && lane <= wz && k <= nb_z;
         const int jl = j < 1 ? 1 : (j > nb_y ? nb_y : j);
         const unsigned int ymask = (j == 0 || j == nb_y + 1) ? kSignMask<TF> : 0u;
         const int kl = k < 1 ? 1 : (k > nb_z ? nb_z : k);
@@ -314,7 +315,7 @@ void blast_columns(const hydro_batch* b, const BatchBlast& B, int& ncol)
         const int nwin = (B.nb_y + kOwned - 1) / kOwned;
         ncol = (nwin + b->group - 1) / b->group;
     } else {
-        const int nzw = (B.nb_z + kOwnedZ - 1) / kOwnedZ;
+        const int nzw = z_windows(B.nb_z);
         const int nyw = (B.nb_y + kTY - 1) / kTY;
         ncol = nzw * nyw;
     }
