@@ -6,9 +6,10 @@
 // Work decomposition:
 //   * a workgroup covers a (Y x Z) tile of TY rows x wz <= 62 columns (z_window_width:
 //     the columns of a row spread evenly over the fewest windows) and marches along X
-//     over a strip of rows; it has TY + 2 waves, wave w holding Y row j0 - 1 + w
-//     (waves 0 and TY+1 are halo rows: they load and convert their row but update
-//     nothing);
+//     over a strip of rows; it has TY + 2 waves, each holding one Y row j0 - 1 + w
+//     (w = 0 and TY+1 are halo rows: they load and convert their row but update
+//     nothing; tile_row_of_wave places them on two of the four SIMDs, alternating
+//     between co-resident workgroups);
 //   * within a wave, lane l holds Z column k = wz*kw + l (lanes 0 and wz+1 are halo
 //     columns, lanes above wz+1 idle); the Z face (k, k+1) is computed once by lane k
 //     and shuffled to lane k+1;
@@ -72,7 +73,8 @@ __global__ void __launch_bounds__(kBlock3, HYDRO_3D_MIN_WAVES)
     Trip<TS>* __restrict__ MN3 = reinterpret_cast<Trip<TS>*>(uvw_next);
 
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
+    __shared__ int s_simd[kNW];
+    const int w = tile_row_of_wave(s_simd);  // tile row of this wave (halo rows placed by SIMD)
     const unsigned lb = xcd_remap3(blockIdx.x, gridDim.x);
     const unsigned per_strip = static_cast<unsigned>(g.nzw) * static_cast<unsigned>(g.nyw);
     const int strip = static_cast<int>(lb / per_strip);

@@ -170,7 +170,8 @@ __global__ void __launch_bounds__(kBlock3, 4)
     const unsigned wid = xcd_remap3(blockIdx.x, gridDim.x);
     if (wid >= nworkers) return;  // workgroup-uniform
     const int lane = threadIdx.x & (kWave - 1);
-    const int w = threadIdx.x / kWave;
+    __shared__ int s_simd[kNW];
+    const int w = tile_row_of_wave(s_simd);  // tile row of this wave (halo rows placed by SIMD)
     const bool halo_wave = (w == 0) || (w == kNW - 1);
     const TF gamma = static_cast<TF>(cs.gamma), gmo = static_cast<TF>(cs.gmo);
     const TF divgamma = static_cast<TF>(cs.divgamma), K = static_cast<TF>(cs.K);

@@ -78,6 +78,32 @@ __device__ __forceinline__ unsigned xcd_remap3(unsigned b, unsigned n)
     return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
+// Tile row held by the calling wave (0 and kNW-1 are the halo rows).  The waves of a
+// workgroup sit two per SIMD, and a halo wave does a fraction of an owned row's work.
+// The halo rows go to waves on SIMDs 0 and 1 in even workgroup slots of the CU and on
+// SIMDs 2 and 3 in odd ones (HW_ID register), so that co-resident workgroups spread the
+// owned rows evenly over the four SIMDs.  Any outcome is a permutation of the rows, and
+// the march's results do not depend on which wave holds which row.  Every thread of
+// the workgroup must call it (one barrier).
+__device__ __forceinline__ int tile_row_of_wave(int (&s_simd)[kNW])
+{
+    const int hw = static_cast<int>(threadIdx.x) / kWave;
+    const unsigned id = __builtin_amdgcn_s_getreg((31 << 11) | 4);  // HW_REG_HW_ID, bits 31:0
+    const int simd = static_cast<int>((id >> 4) & 3u), slot = static_cast<int>((id >> 16) & 15u);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_simd[hw] = simd;
+    __syncthreads();
+    const int want0 = (slot & 1) * 2, want1 = want0 + 1;
+    int lo = 0;  // first wave on SIMD want0 (wave 0 if none)
+    for (int v = kNW - 1; v >= 0; --v)
+        if (s_simd[v] == want0) lo = v;
+    int hi = lo == kNW - 1 ? kNW - 2 : kNW - 1;  // first other wave on SIMD want1
+    for (int v = kNW - 1; v >= 0; --v)
+        if (v != lo && s_simd[v] == want1) hi = v;
+    if (hw == lo) return 0;
+    if (hw == hi) return kNW - 1;
+    return 1 + hw - (lo < hw ? 1 : 0) - (hi < hw ? 1 : 0);
+}
+
 // Primitive state in the X frame: (rho, u, v, w, p, a).
 template <typename T>
 struct P6 {
