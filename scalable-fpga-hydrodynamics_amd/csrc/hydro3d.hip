@@ -49,8 +49,11 @@ struct Geom3 {
 
 // TS: state and update precision; TF, M: precision and math policy of the primitives,
 // fluxes and signal speeds (see the 2D kernel).
+template <typename TS>
+constexpr int kMinWaves3 = std::is_same_v<TS, float> ? HYDRO_3D_MIN_WAVES_FP32 : HYDRO_3D_MIN_WAVES;
+
 template <typename TS, typename TF, typename M, bool WALL_Y, bool WALL_Z, bool RESIDENT>
-__global__ void __launch_bounds__(kBlock3, HYDRO_3D_MIN_WAVES)
+__global__ void __launch_bounds__(kBlock3, (kMinWaves3<TS>))
     fvm3d_kernel(const TS* __restrict__ rhoE, const TS* __restrict__ uvw, TS* __restrict__ rhoE_next,
                  TS* __restrict__ uvw_next, Geom3 g, Consts3<TS> cs, TS DtDx_arg, TS DtDy_arg, TS DtDz_arg,
                  StepReduce* red, StepControl* ctl, TS* Dt_next, double* traj_t, TS* traj_dt)

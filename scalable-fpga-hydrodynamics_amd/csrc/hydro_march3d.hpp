@@ -40,9 +40,14 @@ __host__ __device__ __forceinline__ int z_window_width(int nb_z)
 #ifndef HYDRO_3D_PREFETCH
 #define HYDRO_3D_PREFETCH 1
 #endif
-// Minimum waves per SIMD of the 3D step kernel (4: two 8-wave workgroups per CU).
+// Minimum waves per SIMD of the 3D step kernel: 4 (two 8-wave workgroups per CU, at
+// most 128 VGPRs) for FP64 state, 8 (four workgroups, at most 64 VGPRs) for FP32 state,
+// whose step is latency-bound and runs 18% slower at two workgroups per CU than at three.
 #ifndef HYDRO_3D_MIN_WAVES
 #define HYDRO_3D_MIN_WAVES 4
+#endif
+#ifndef HYDRO_3D_MIN_WAVES_FP32
+#define HYDRO_3D_MIN_WAVES_FP32 8
 #endif
 constexpr int kTY = HYDRO_3D_TY;     // owned Y rows per workgroup
 constexpr int kPF3 = HYDRO_3D_PREFETCH;
