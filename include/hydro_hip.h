@@ -45,6 +45,12 @@ extern "C" {
 
 /* ---------------------------------------------------------------------------------
  * Drop-in launchers.
+ *
+ * Concurrency (launcher*, hydro_step2d, hydro_step3d): steps on different streams, or
+ * from different host threads, may overlap on the device.  Each stream of each device
+ * (each thread, for hipStreamPerThread) has its own Dt reduction state: 512 bytes of
+ * device memory, allocated at its first step and kept until the process ends.  Steps on
+ * one stream run in stream order, as any HIP work does.
  * ------------------------------------------------------------------------------- */
 
 /* Replaces `launcher` of hydro2d/kernel.hpp:63-68 (defined at hydro2d/kernel.cxx:331-367).

@@ -34,10 +34,6 @@
 
 namespace hydro {
 
-// Reduction state of the plain (non-resident) launchers, one per device and precision
-// ([0] FP64, [1] FP32); its device address is looked up with hipGetSymbolAddress.
-__device__ StepReduce g_red2d[2];
-
 namespace {
 
 struct Geom2 {
@@ -277,19 +273,6 @@ int launch_ghost2d(T* rhoE, T* uv, size_t nb_x, size_t nb_y, int bc_x, int bc_y,
 }
 
 
-StepReduce* plain_reduce_state(int which)
-{
-    static StepReduce* addr[64] = {};
-    const int dev = current_device();
-    if (dev < 0 || dev >= 64) return nullptr;
-    if (!addr[dev]) {
-        void* p = nullptr;
-        HYDRO_HIP_OR(hipGetSymbolAddress(&p, HIP_SYMBOL(g_red2d)), nullptr);
-        addr[dev] = static_cast<StepReduce*>(p);
-    }
-    return addr[dev] + which;
-}
-
 struct EventPair {
     hipEvent_t a = nullptr, b = nullptr;
 };
@@ -313,7 +296,7 @@ double timed_launcher2d(const T* d_rhoE, const T* d_uv, T* d_rhoE_next, T* d_uv_
 {
     clear_error();
     hipStream_t s = static_cast<hipStream_t>(stream);
-    StepReduce* red = plain_reduce_state(sizeof(T) == 8 ? 0 : 1);
+    StepReduce* red = plain_reduce_state(s);
     EventPair* ev = timing_events();
     if (!red || !ev) return HYDRO_ERR_HIP;
     const Consts<T> c{ C_min, gamma, gamma_minus_one, divgamma, K };
@@ -406,7 +389,7 @@ extern "C" int hydro_step2d(const double* d_rhoE, const double* d_uv, double* d_
                             double K, size_t NB_X, size_t NB_Y, double DtDx, double DtDy, int bc_y, void* stream)
 {
     clear_error();
-    StepReduce* red = plain_reduce_state(0);
+    StepReduce* red = plain_reduce_state(static_cast<hipStream_t>(stream));
     if (!red) return HYDRO_ERR_HIP;
     const Consts<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };
     return launch_step2d<double>(d_rhoE, d_uv, d_rhoE_next, d_uv_next, c, NB_X, NB_Y, DtDx, DtDy, bc_y, red, nullptr,

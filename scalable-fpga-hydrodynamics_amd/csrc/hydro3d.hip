@@ -32,10 +32,6 @@
 
 namespace hydro {
 
-// Reduction state of the plain (non-resident) launchers, one per device and precision
-// ([0] FP64, [1] FP32); its device address is looked up with hipGetSymbolAddress.
-__device__ StepReduce g_red3d[2];
-
 namespace {
 
 struct Geom3 {
@@ -301,19 +297,6 @@ int launch_ghost3d(T* rhoE, T* uvw, size_t nb_x, size_t nb_y, size_t nb_z, int b
 }
 
 
-StepReduce* plain_reduce_state3(int which)
-{
-    static StepReduce* addr[64] = {};
-    const int dev = current_device();
-    if (dev < 0 || dev >= 64) return nullptr;
-    if (!addr[dev]) {
-        void* p = nullptr;
-        HYDRO_HIP_OR(hipGetSymbolAddress(&p, HIP_SYMBOL(g_red3d)), nullptr);
-        addr[dev] = static_cast<StepReduce*>(p);
-    }
-    return addr[dev] + which;
-}
-
 struct EventPair3 {
     hipEvent_t a = nullptr, b = nullptr;
 };
@@ -338,7 +321,7 @@ double timed_launcher3d(const T* d_rhoE, const T* d_uvw, T* d_rhoE_next, T* d_uv
 {
     clear_error();
     hipStream_t s = static_cast<hipStream_t>(stream);
-    StepReduce* red = plain_reduce_state3(sizeof(T) == 8 ? 0 : 1);
+    StepReduce* red = plain_reduce_state(s);
     EventPair3* ev = timing_events3();
     if (!red || !ev) return HYDRO_ERR_HIP;
     const Consts3<T> c{ C_min, gamma, gamma_minus_one, divgamma, K };
@@ -425,7 +408,7 @@ extern "C" int hydro_step3d(const double* d_rhoE, const double* d_uvw, double* d
                             int bc_y, int bc_z, void* stream)
 {
     clear_error();
-    StepReduce* red = plain_reduce_state3(0);
+    StepReduce* red = plain_reduce_state(static_cast<hipStream_t>(stream));
     if (!red) return HYDRO_ERR_HIP;
     const Consts3<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };
     return launch_step3d<double>(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, c, NB_X, NB_Y, NB_Z, DtDx, DtDy, DtDz, bc_y,

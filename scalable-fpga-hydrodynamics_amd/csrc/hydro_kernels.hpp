@@ -12,6 +12,11 @@ namespace hydro {
 // kernel, so that no ghost fill is needed for them (bits of `wraps`).
 constexpr int kWrapX = 1, kWrapY = 2, kWrapZ = 4;
 
+// Reduction state of the plain launchers on stream s of the current device (one per
+// stream, so that plain steps on different streams can overlap); NULL with the error
+// recorded on failure.
+StepReduce* plain_reduce_state(hipStream_t s);
+
 // consts = { C_min, gamma, gamma_minus_one, divgamma, K } in the state precision.  When
 // ev_a / ev_b are given they are recorded around the step kernel (not the finalise).
 int step2d_resident(const void* rhoE, const void* mom, void* rhoE_next, void* mom_next, size_t nb_x, size_t nb_y,

@@ -5,11 +5,16 @@
 
 #include <stdarg.h>
 #include <stdio.h>
+#include <string.h>
 
 #include <exception>
+#include <map>
+#include <mutex>
 #include <new>
+#include <thread>
 
 #include "../../include/hydro_hip.h"
+#include "hydro_kernels.hpp"
 
 namespace {
 thread_local char g_last_error[512] = "";
@@ -51,15 +56,58 @@ int current_device()
     return dev;
 }
 
+// Reduction state of the plain launchers (launcher*, hydro_step*): one per (device,
+// stream), and per thread for the per-thread default stream, whose handle every thread
+// shares.  Steps on one stream run in order and can share a state; steps on different
+// streams may overlap on the device and must not (their atomic maxima would mix, and one
+// finalize would clear the other's slots).  Each state is sizeof(StepReduce) bytes of
+// device memory, zeroed once here and cleared by the finalize kernel after every step,
+// and is kept until the process ends.
+StepReduce* plain_reduce_state(hipStream_t s)
+{
+    struct Key {
+        int dev;
+        hipStream_t s;
+        std::thread::id t;
+        bool operator<(const Key& o) const
+        {
+            if (dev != o.dev) return dev < o.dev;
+            if (s != o.s) return std::less<hipStream_t>()(s, o.s);
+            return t < o.t;
+        }
+    };
+    static std::mutex mu;
+    static std::map<Key, StepReduce*> states;
+    const int dev = current_device();
+    if (dev < 0) return nullptr;
+    const Key key{ dev, s, s == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id() };
+    std::lock_guard<std::mutex> lk(mu);
+    const auto it = states.find(key);
+    if (it != states.end()) return it->second;
+    StepReduce* p = nullptr;
+    HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&p), sizeof(StepReduce)), nullptr);
+    // A synchronous fill, not an async one on s: a caller may be capturing s in a graph.
+    hipError_t e = hipMemset(p, 0, sizeof(StepReduce));
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        set_error("reduction state: hipMemset failed: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    try {
+        states.emplace(key, p);
+    } catch (...) {
+        (void)hipFree(p);
+        set_error("reduction state: out of host memory");
+        return nullptr;
+    }
+    return p;
+}
+
 }  // namespace hydro
 
 extern "C" const char* hydro_last_error(void) { return g_last_error; }
 
 extern "C" int hydro_version(void) { return HYDRO_HIP_ABI_VERSION; }
-
-#include <string.h>
-
-#include "hydro_kernels.hpp"
 
 extern "C" int hydro_tune_set(const char* key, int value)
 {

@@ -1,0 +1,113 @@
+"""Plain steps of two solves that overlap on the device.
+
+The plain entry points (launcher, launcher3d, hydro_step2d, hydro_step3d) reduce the new
+signal speeds into a state kept per stream: two solves enqueued on different streams,
+or launched from two host threads, may run at the same time and must each get the Dt
+and fields they get alone.  Solve B is given 40x the energy of solve A, so any mixing of
+the two reductions would change A's Dt.
+"""
+import threading
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_helpers import assert_bitwise, random_state_2d, random_state_3d, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+DTDX = 0.02  # keeps every updated state physical (see test_gpu_parity.py)
+
+
+def _pair(make, sizes, seeds):
+    out = []
+    for nb, seed, scale in zip(sizes, seeds, (1.0, 40.0)):
+        rhoE, mom = make(*nb, seed=seed)
+        rhoE[..., 1] *= scale
+        out.append((nb, to_dev(rhoE), to_dev(mom)))
+    return out
+
+
+def _outputs(rhoE, mom):
+    return torch.full_like(rhoE, -1.0), torch.full_like(mom, -1.0), torch.zeros(1, dtype=torch.float64, device="cuda")
+
+
+def _step(gpu, dim, nb, r, m, out, stream):
+    k = gpu.constants(nb, 0.5 if dim == 2 else 0.4)
+    n_r, n_m, dt = out
+    if dim == 2:
+        gpu.step2d(r, m, n_r, n_m, dt, k["C_min"], k["gamma"], k["gmo"], k["divgamma"], k["K"], DTDX, DTDX,
+                   stream=stream)
+    else:
+        gpu.step3d(r, m, n_r, n_m, dt, k["C_min"], k["gamma"], k["gmo"], k["divgamma"], k["K"], DTDX, DTDX, DTDX,
+                   stream=stream)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_steps_on_two_streams_match_each_alone(gpu, dim):
+    if dim == 2:
+        solves = _pair(random_state_2d, [(1500, 1000), (1200, 1100)], [11, 12])
+    else:
+        solves = _pair(random_state_3d, [(120, 100, 90), (100, 110, 80)], [13, 14])
+    alone = []
+    for nb, r, m in solves:
+        out = _outputs(r, m)
+        _step(gpu, dim, nb, r, m, out, None)
+        alone.append([to_host(t) for t in out])
+    assert alone[0][2][0] != alone[1][2][0]
+
+    cur = torch.cuda.current_stream()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [_outputs(r, m) for _, r, m in solves]
+    for s in streams:
+        s.wait_stream(cur)
+    for _ in range(30):  # the same step again and again, alternating, so the two overlap
+        for (nb, r, m), out, s in zip(solves, outs, streams):
+            _step(gpu, dim, nb, r, m, out, s)
+    torch.cuda.synchronize()
+    for name, want, out in zip("AB", alone, outs):
+        got = [to_host(t) for t in out]
+        assert_bitwise(got[2], want[2], f"{dim}D solve {name}: Dt_next")
+        assert_bitwise(got[0], want[0], f"{dim}D solve {name}: rhoE_next")
+        assert_bitwise(got[1], want[1], f"{dim}D solve {name}: momentum_next")
+
+
+def test_launcher_from_two_threads_matches_each_alone(gpu):
+    solves = _pair(random_state_2d, [(1500, 1000), (1200, 1100)], [21, 22])
+    consts = [gpu.constants(nb, 0.5) for nb, _, _ in solves]
+
+    def launch(i, out, stream):
+        (nb, r, m), k = solves[i], consts[i]
+        n_r, n_m, dt = out
+        gpu.launcher(r, m, n_r, n_m, dt, k["C_min"], k["gamma"], k["gmo"], k["divgamma"], k["K"], DTDX, DTDX,
+                     stream=stream)
+
+    alone = []
+    for i, (_, r, m) in enumerate(solves):
+        out = _outputs(r, m)
+        launch(i, out, None)
+        alone.append([to_host(t) for t in out])
+
+    outs = [_outputs(r, m) for _, r, m in solves]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    errors = []
+
+    def worker(i):
+        try:
+            for _ in range(20):
+                launch(i, outs[i], streams[i])
+        except Exception as e:  # noqa: BLE001  (reported by the main thread)
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    torch.cuda.synchronize()
+    for name, want, out in zip("AB", alone, outs):
+        got = [to_host(t) for t in out]
+        assert_bitwise(got[2], want[2], f"launcher solve {name}: Dt_next")
+        assert_bitwise(got[0], want[0], f"launcher solve {name}: rhoE_next")
