@@ -56,13 +56,15 @@ int current_device()
     return dev;
 }
 
+__global__ void clear_reduce_kernel(StepReduce* red) { red->slot[threadIdx.x] = 0ull; }
+
 // Reduction state of the plain launchers (launcher*, hydro_step*): one per (device,
 // stream), and per thread for the per-thread default stream, whose handle every thread
 // shares.  Steps on one stream run in order and can share a state; steps on different
 // streams may overlap on the device and must not (their atomic maxima would mix, and one
 // finalize would clear the other's slots).  Each state is sizeof(StepReduce) bytes of
 // device memory, zeroed once here and cleared by the finalize kernel after every step,
-// and is kept until the process ends.
+// and is kept until the process ends.  The caller enqueues its step on s right after.
 StepReduce* plain_reduce_state(hipStream_t s)
 {
     struct Key {
@@ -86,11 +88,14 @@ StepReduce* plain_reduce_state(hipStream_t s)
     if (it != states.end()) return it->second;
     StepReduce* p = nullptr;
     HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&p), sizeof(StepReduce)), nullptr);
-    // A synchronous fill, not an async one on s: a caller may be capturing s in a graph.
-    hipError_t e = hipMemset(p, 0, sizeof(StepReduce));
+    // Cleared on s itself, so that the clear is ordered before the first step on s; by a
+    // kernel rather than a memset, which a caller capturing s would record as a memset
+    // node.
+    hipLaunchKernelGGL(clear_reduce_kernel, dim3(1), dim3(kRedSlots), 0, s, p);
+    const hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         (void)hipFree(p);
-        set_error("reduction state: hipMemset failed: %s", hipGetErrorString(e));
+        set_error("reduction state: clear kernel launch failed: %s", hipGetErrorString(e));
         return nullptr;
     }
     try {
