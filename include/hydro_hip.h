@@ -48,9 +48,13 @@ extern "C" {
  *
  * Concurrency (launcher*, hydro_step2d, hydro_step3d): steps on different streams, or
  * from different host threads, may overlap on the device.  Each stream of each device
- * (each thread, for hipStreamPerThread) has its own Dt reduction state: 512 bytes of
- * device memory, allocated at its first step and kept until the process ends.  Steps on
- * one stream run in stream order, as any HIP work does.
+ * (each thread, for hipStreamPerThread) has its own 512-byte Dt reduction state, bound
+ * to it at its first step and kept until the process ends.  The states come from a pool
+ * of 256 per device in the library's device image, so the launchers allocate nothing;
+ * a device's 257th and later streams get a hipMalloc'd state.  Steps on one stream run
+ * in stream order, as any HIP work does.  A new stream that reuses the handle of a
+ * destroyed one takes over its state, so let a stream's steps finish before destroying
+ * it.
  * ------------------------------------------------------------------------------- */
 
 /* Replaces `launcher` of hydro2d/kernel.hpp:63-68 (defined at hydro2d/kernel.cxx:331-367).

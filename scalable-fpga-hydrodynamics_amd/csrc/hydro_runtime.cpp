@@ -58,13 +58,20 @@ int current_device()
 
 __global__ void clear_reduce_kernel(StepReduce* red) { red->slot[threadIdx.x] = 0ull; }
 
+// Pool of reduction states of the plain launchers, one per stream that has stepped on
+// the device; part of the library's device image, so the launchers allocate nothing
+// for the first kPlainStates streams of a device.
+constexpr int kPlainStates = 256;
+__device__ StepReduce g_plain_red[kPlainStates];
+
 // Reduction state of the plain launchers (launcher*, hydro_step*): one per (device,
 // stream), and per thread for the per-thread default stream, whose handle every thread
 // shares.  Steps on one stream run in order and can share a state; steps on different
 // streams may overlap on the device and must not (their atomic maxima would mix, and one
-// finalize would clear the other's slots).  Each state is sizeof(StepReduce) bytes of
-// device memory, zeroed once here and cleared by the finalize kernel after every step,
-// and is kept until the process ends.  The caller enqueues its step on s right after.
+// finalize would clear the other's slots).  A state is taken from g_plain_red, or
+// allocated (sizeof(StepReduce) bytes) once a device's pool is used up, and is kept until
+// the process ends.  It is cleared here on s, and by the finalize kernel after every
+// step; the caller enqueues its step on s right after.
 StepReduce* plain_reduce_state(hipStream_t s)
 {
     struct Key {
@@ -78,8 +85,13 @@ StepReduce* plain_reduce_state(hipStream_t s)
             return t < o.t;
         }
     };
+    struct Pool {
+        StepReduce* base = nullptr;
+        int used = 0;
+    };
     static std::mutex mu;
     static std::map<Key, StepReduce*> states;
+    static std::map<int, Pool> pools;
     const int dev = current_device();
     if (dev < 0) return nullptr;
     const Key key{ dev, s, s == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id() };
@@ -87,21 +99,32 @@ StepReduce* plain_reduce_state(hipStream_t s)
     const auto it = states.find(key);
     if (it != states.end()) return it->second;
     StepReduce* p = nullptr;
-    HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&p), sizeof(StepReduce)), nullptr);
-    // Cleared on s itself, so that the clear is ordered before the first step on s; by a
-    // kernel rather than a memset, which a caller capturing s would record as a memset
-    // node.
-    hipLaunchKernelGGL(clear_reduce_kernel, dim3(1), dim3(kRedSlots), 0, s, p);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        (void)hipFree(p);
-        set_error("reduction state: clear kernel launch failed: %s", hipGetErrorString(e));
-        return nullptr;
-    }
+    bool allocated = false;
     try {
+        Pool& pool = pools[dev];
+        if (!pool.base) {
+            void* a = nullptr;
+            HYDRO_HIP_OR(hipGetSymbolAddress(&a, HIP_SYMBOL(g_plain_red)), nullptr);
+            pool.base = static_cast<StepReduce*>(a);
+        }
+        if (pool.used < kPlainStates) {
+            p = pool.base + pool.used++;
+        } else {
+            HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&p), sizeof(StepReduce)), nullptr);
+            allocated = true;
+        }
+        // On s, so that the clear is ordered before the first step on s; by a kernel
+        // rather than a memset, which a caller capturing s would record as a memset node.
+        hipLaunchKernelGGL(clear_reduce_kernel, dim3(1), dim3(kRedSlots), 0, s, p);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error("reduction state: clear kernel launch failed: %s", hipGetErrorString(e));
+            if (allocated) (void)hipFree(p);
+            return nullptr;
+        }
         states.emplace(key, p);
     } catch (...) {
-        (void)hipFree(p);
+        if (allocated) (void)hipFree(p);
         set_error("reduction state: out of host memory");
         return nullptr;
     }

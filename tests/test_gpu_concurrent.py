@@ -111,3 +111,42 @@ def test_launcher_from_two_threads_matches_each_alone(gpu):
         got = [to_host(t) for t in out]
         assert_bitwise(got[2], want[2], f"launcher solve {name}: Dt_next")
         assert_bitwise(got[0], want[0], f"launcher solve {name}: rhoE_next")
+
+
+def test_more_streams_than_the_state_pool(gpu):
+    """300 distinct streams, more than the 256 pooled reduction states of a device: the
+    later streams get allocated states, and every stream's step gives the Dt and fields
+    of the step alone."""
+    import ctypes
+
+    # The HIP runtime calls are resolved through the solver library's own handle, so they
+    # reach the runtime it (and torch) use, never a second copy.
+    hip = gpu.lib()
+    hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+    hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    nb = (64, 48)
+    rhoE, uv = random_state_2d(*nb, seed=31)
+    r, m = to_dev(rhoE), to_dev(uv)
+    out = _outputs(r, m)
+    _step(gpu, 2, nb, r, m, out, None)
+    want = [to_host(t) for t in out]
+    streams = []
+    try:
+        for _ in range(300):
+            h = ctypes.c_void_p()
+            assert hip.hipStreamCreateWithFlags(ctypes.byref(h), 1) == 0  # hipStreamNonBlocking
+            streams.append(h)
+        torch.cuda.synchronize()
+        for i, h in enumerate(streams):
+            got_out = _outputs(r, m)
+            torch.cuda.synchronize()
+            _step(gpu, 2, nb, r, m, got_out, h.value)
+            assert hip.hipStreamSynchronize(h.value) == 0
+            got = [to_host(t) for t in got_out]
+            assert_bitwise(got[2], want[2], f"stream {i}: Dt_next")
+            assert_bitwise(got[0], want[0], f"stream {i}: rhoE_next")
+    finally:
+        torch.cuda.synchronize()
+        for h in streams:
+            hip.hipStreamDestroy(h.value)
