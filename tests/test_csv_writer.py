@@ -79,3 +79,30 @@ def test_csv_writer_matches_printf(harness, tmp_path, dim, nb, threads):
     want = (tmp_path / "want.csv").read_bytes()
     assert got.count(b"\n") == nb[0] * nb[1] * (nb[2] if dim == 3 else 1) + 1
     assert got == want
+
+
+@pytest.fixture(scope="module")
+def harness_sanitized(tmp_path_factory):
+    d = tmp_path_factory.mktemp("csv_san")
+    src = d / "h.cpp"
+    src.write_text(HARNESS)
+    exe = d / "h"
+    subprocess.run(["g++", "-O1", "-g", "-fno-omit-frame-pointer", "-std=c++17", "-pthread",
+                    "-fsanitize=address,undefined", "-fno-sanitize-recover=all", f"-I{HOST}", str(src),
+                    str(HOST / "hydro_io.cpp"), "-o", str(exe)], check=True, capture_output=True)
+    return exe
+
+
+def test_csv_writer_clean_under_asan_and_ubsan(harness_sanitized, tmp_path):
+    """Several 131072-row blocks formatted by 8 threads, under AddressSanitizer and
+    UndefinedBehaviorSanitizer: the same bytes as printf and no report."""
+    import os
+
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:verify_asan_link_order=0",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([str(harness_sanitized), "2", "400", "900", "1", "8"], cwd=tmp_path, env=env,
+                       capture_output=True, text=True, timeout=300)
+    report = r.stdout + r.stderr
+    assert r.returncode == 0, report
+    assert "Sanitizer" not in report and "runtime error" not in report, report
+    assert (tmp_path / "got.csv").read_bytes() == (tmp_path / "want.csv").read_bytes()
