@@ -46,6 +46,7 @@ CASES = [
     ("hydro", (70, 33), 0.01, 1000, 5, [], np.float64, None),
     ("hydro3d", (24, 20, 16), 2.0, 40, 15, [], np.float64, None),
     ("hydro3d", (16, 12, 20), 2.0, 30, 10, ["--sync-steps", "--bcz", "periodic"], np.float64, (0, 1, 0)),
+    ("hydro3d", (22, 18, 26), 2.0, 30, 10, ["--precision", "fp32"], np.float32, None),
     # Several 131072-row blocks per file (parallel formatting, in-order writes) and a
     # file written in the background while the next steps run.
     ("hydro", (700, 400), 2.0, 12, 5, [], np.float64, None),

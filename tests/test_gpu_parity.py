@@ -266,7 +266,7 @@ def test_launch_geometry_keeps_results_2d(gpu, precision):
         assert_bitwise(got[1], base[1], f"uv rows {rows} occ {occ}")
 
 
-@pytest.mark.parametrize("precision", [64, 3264])
+@pytest.mark.parametrize("precision", [64, 3264, 32])
 def test_launch_geometry_keeps_results_3d(gpu, precision):
     nb = (300, 256, 256)
     base, last = _geometry_run(gpu, nb, precision, 6, {"rows3d": 0})
