@@ -927,11 +927,23 @@ try {
         for (int bpc = 1; bpc <= max_blocks(b); ++bpc)
             for (int r : { 0, 4, 16 }) cands.push_back({ bpc, r, 1 });
     }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    HYDRO_HIP_OR(hipEventCreate(&e0), HYDRO_ERR_HIP);
-    HYDRO_HIP_OR(hipEventCreate(&e1), HYDRO_ERR_HIP);
-    const size_t saved_graph = b->graph_steps;
-    b->graph_steps = 0;  // trials use direct launches
+    // Trials use direct launches; the graph setting and the events are restored and
+    // released on every way out, errors and exceptions included.
+    struct TrialScope {
+        hydro_batch* b;
+        size_t graph_steps;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~TrialScope()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            b->graph_steps = graph_steps;
+        }
+    } scope{ b, b->graph_steps };
+    b->graph_steps = 0;
+    HYDRO_HIP_OR(hipEventCreate(&scope.e0), HYDRO_ERR_HIP);
+    HYDRO_HIP_OR(hipEventCreate(&scope.e1), HYDRO_ERR_HIP);
+    hipEvent_t e0 = scope.e0, e1 = scope.e1;
     auto trial = [&](const TuneChoice& c, double& per) -> int {
         int rc = hydro_batch_set_tiling(b, c.bpc, c.rows, c.group);
         if (rc == 0 && b->dirty) rc = build_tables(b);
@@ -971,9 +983,6 @@ try {
             }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    b->graph_steps = saved_graph;
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
