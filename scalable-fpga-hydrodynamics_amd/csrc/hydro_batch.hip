@@ -642,9 +642,14 @@ try {
     b->dim = dim;
     b->precision = precision;
     b->cap = capacity;
-    b->h_blasts.assign(capacity, BatchBlast{});
-    b->used.assign(capacity, 0);
-    b->h_done.assign(capacity, 0);
+    try {
+        b->h_blasts.assign(capacity, BatchBlast{});
+        b->used.assign(capacity, 0);
+        b->h_done.assign(capacity, 0);
+    } catch (...) {
+        delete b;  // nothing on the device yet
+        throw;
+    }
     const double gamma = 1.4, K = 1.1, gmo = gamma - 1.0;  // hydro2d/main.cxx:601-606
     b->cs.C_min = 0.0;
     b->cs.gamma = gamma;

@@ -161,22 +161,39 @@ int ensure_engine(hydro_server* s, int e)
     if (s->eng[e]) return 0;
     const int dim = e >= 2 ? 3 : 2;
     const int prec = (e & 1) ? HYDRO_FP32_FLUX : HYDRO_FP64;
-    s->eng[e] = hydro_batch_create(dim, prec, s->capacity);
-    if (!s->eng[e]) return HYDRO_ERR_HIP;
-    s->slot_req[e].assign(s->capacity, -1);
-    if (hipEventCreateWithFlags(&s->ev[e], hipEventDisableTiming) != hipSuccess) {
-        set_error("hydro server: cannot create an event");
-        return HYDRO_ERR_HIP;
-    }
+    // Built in locals and published only once complete, so that a failure leaves no
+    // half-configured engine for the next call to take as ready.
+    hydro_batch* b = hydro_batch_create(dim, prec, s->capacity);
+    if (!b) return HYDRO_ERR_HIP;
     // Continuous batching replays a few steps at a time from a graph.  The balanced
     // tiling (equal row ranges per worker of a persistent grid) stays efficient whatever
     // the mix, down to a single request; measured on the served mix, it beats direct
     // dispatch of R-row tiles (p50 57 vs 71-83 ms).  Two workgroups per CU per engine:
     // with four engines sharing the device, three per CU for the 2D engines gave a p50
     // of 58 ms against 52 ms at the same throughput.
-    int rc = hydro_batch_set_graph(s->eng[e], 10);
-    if (rc == 0) rc = hydro_batch_set_tiling(s->eng[e], 2, 0, 1);
-    return rc;
+    int rc = hydro_batch_set_graph(b, 10);
+    if (rc == 0) rc = hydro_batch_set_tiling(b, 2, 0, 1);
+    hipEvent_t ev = nullptr;
+    if (rc == 0 && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        set_error("hydro server: cannot create an event");
+        rc = HYDRO_ERR_HIP;
+    }
+    if (rc == 0) {
+        try {
+            s->slot_req[e].assign(s->capacity, -1);
+        } catch (...) {
+            set_error("hydro server: out of host memory");
+            rc = HYDRO_ERR_NOMEM;
+        }
+    }
+    if (rc != 0) {
+        if (ev) (void)hipEventDestroy(ev);
+        hydro_batch_destroy(b);
+        return rc;
+    }
+    s->eng[e] = b;
+    s->ev[e] = ev;
+    return 0;
 }
 
 // Admit queued requests in arrival order.  A request whose engine is busy (mid-tick) or
@@ -285,7 +302,12 @@ try {
         delete s;
         return nullptr;
     }
-    s->arena.reset(hbm_bytes);
+    try {
+        s->arena.reset(hbm_bytes);
+    } catch (...) {
+        hydro_server_destroy(s);  // frees the arena's device memory and the server
+        throw;
+    }
     return s;
 } catch (...) {
     hydro::set_exception_error(__func__);
