@@ -76,6 +76,12 @@ try {
         set_error("hydro_run_create: bad dim/precision/pointer arguments");
         return nullptr;
     }
+    // The trajectory arrays hold max_it entries of 8 bytes; a count whose size overflows
+    // would give a short buffer that the steps then write past.
+    if (max_it > SIZE_MAX / sizeof(double)) {
+        set_error("hydro_run_create: max_it %zu is too large for its trajectory arrays", max_it);
+        return nullptr;
+    }
     for (int d = 0; d < dim; ++d) {
         if (nb[d] < 1) {
             set_error("hydro_run_create: nb[%d] must be >= 1", d);

@@ -62,6 +62,9 @@ def test_bad_arguments_are_rejected_without_a_device(hydro):
     bad_bc = (C.c_int * 3)(0, 7, 0)
     assert L.hydro_run_create(2, sizes, D, 0.05, 1e-3, 1.0, 10, 64, bad_bc) is None
     assert "boundary" in hydro.last_error()
+    # max_it whose trajectory size would overflow (2^61 x 8 bytes wraps to 0)
+    assert L.hydro_run_create(2, sizes, D, 0.05, 1e-3, 1.0, 1 << 61, 64, None) is None
+    assert "max_it" in hydro.last_error()
     with pytest.raises(hydro.HydroError):
         hydro.tune_set("no_such_knob", 3)
     with pytest.raises(hydro.HydroError):
