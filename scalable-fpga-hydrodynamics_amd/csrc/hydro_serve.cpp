@@ -358,12 +358,24 @@ try {
     r.max_it = max_it;
     r.keep = keep_result != 0;
     r.t_arrival = arrival >= 0 ? arrival : s->now();
-    const size_t cells = cells_of(r);
+    // The request's footprint, computed with overflow checks: a product that wrapped
+    // could pass the arena check below with a short buffer that the kernels overrun.
+    // Each of the four arrays starts on a 256-byte boundary of the arena.
+    size_t cells = 1;
+    bool wrapped = false;
+    for (int d = 0; d < dim; ++d) wrapped |= __builtin_mul_overflow(cells, r.nb[d] + 2, &cells);
+    size_t b0 = 0, b1 = 0;
+    wrapped |= __builtin_mul_overflow(cells, size_t(2 * 8), &b0);
+    wrapped |= __builtin_mul_overflow(cells, static_cast<size_t>(dim) * 8, &b1);
+    if (wrapped || b0 > SIZE_MAX / 8 || b1 > SIZE_MAX / 8) {
+        set_error("hydro_server_submit: request of %zu x %zu x %zu cells is too large", r.nb[0], r.nb[1],
+                  dim == 3 ? r.nb[2] : size_t(1));
+        return HYDRO_ERR_NOMEM;
+    }
     r.interior = 1;
     for (int d = 0; d < dim; ++d) r.interior *= static_cast<double>(r.nb[d]);
-    // Each of the four arrays starts on a 256-byte boundary of the arena.
-    r.buf_bytes[0] = (cells * 2 * 8 + kAlign - 1) / kAlign * kAlign;
-    r.buf_bytes[1] = (cells * static_cast<size_t>(dim) * 8 + kAlign - 1) / kAlign * kAlign;
+    r.buf_bytes[0] = (b0 + kAlign - 1) / kAlign * kAlign;
+    r.buf_bytes[1] = (b1 + kAlign - 1) / kAlign * kAlign;
     r.bytes = 2 * (r.buf_bytes[0] + r.buf_bytes[1]);
     if (r.bytes > s->arena.size()) {
         set_error("hydro_server_submit: request needs %zu bytes, more than the server's %zu", r.bytes,

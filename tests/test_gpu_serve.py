@@ -104,6 +104,20 @@ def test_admission_respects_the_hbm_budget(gpu):
     s.close()
 
 
+def test_submit_refuses_sizes_whose_footprint_overflows(gpu):
+    """Footprints that wrap a 64-bit byte count to 0 ((2^30-2)^2 cells in 2D,
+    (2^30-2)^2 x 14 in 3D, 16 bytes each) are refused at submission; nothing is
+    admitted and no step runs."""
+    s = gpu.Server(1 << 30, capacity=4, steps_per_tick=10)
+    try:
+        for nb in ((2**30 - 2, 2**30 - 2), (2**30 - 2, 2**30 - 2, 14)):
+            with pytest.raises(gpu.HydroError, match="too large"):
+                s.submit(nb, gpu.FP64)
+        assert s.pending() == (0, 0, 0)
+    finally:
+        s.close()
+
+
 def test_trace_replay_reports_latency(gpu):
     reqs = workloads.served_requests(10, rate_per_s=200.0, seed=9, side2d=(64, 256), side3d=(16, 48), steps=(20, 80))
     s = gpu.Server(1 << 30, capacity=32, steps_per_tick=10)
