@@ -374,6 +374,18 @@ def bench_batched(args, name: str) -> dict:
     }
 
 
+def _latency_by_dim(reqs, lat) -> dict:
+    """p50 / p99 of the latencies of the 2D and of the 3D requests separately."""
+    sv = importlib.import_module("scalable-fpga-hydrodynamics_amd.serve")
+    out = {}
+    for name, dim in (("2d", 2), ("3d", 3)):
+        xs = [x for r, x in zip(reqs, lat) if len(r["nb"]) == dim]
+        if xs:
+            out[name] = {"requests": len(xs), "p50_s": round(sv.percentile(xs, 50), 4),
+                         "p99_s": round(sv.percentile(xs, 99), 4)}
+    return out
+
+
 def bench_served(args) -> dict:
     """Poisson stream of mixed solves (70% 2D sides 128..4096, 30% 3D sides 64..256,
     100..2000 steps, FP32-flux / FP64 50/50, seeded blasts) through the continuous-batching
@@ -449,6 +461,8 @@ def bench_served(args) -> dict:
                                      "p99_s": round(sv.percentile(sync_lat, 99), 4),
                                      "gcups": round(work / (sync_finish - reqs[0]["arrival"]) / 1e9, 3),
                                      "mean_solve_s": round(sum(dur_sync) / n, 4)},
+        "by_dimension": {"served": _latency_by_dim(reqs, live["latency"]),
+                         "sequential_plain": _latency_by_dim(reqs, seq_lat)},
         "saturation": {"served_gcups": round(sat["gcups"], 3), "sequential_plain_gcups": round(seq_gcups, 3),
                        "speedup": round(sat["gcups"] / seq_gcups, 3),
                        "sequential_sync_launcher_gcups": round(work / sum(dur_sync) / 1e9, 3),
@@ -741,7 +755,8 @@ def main() -> int:
             "served_mixed": {"requests": sv["config"]["requests"], "arrival_rate_per_s": sv["config"]["arrival_rate_per_s"],
                              "load": sv["config"]["load"], "served": sv["served"],
                              "sequential_plain": sv["sequential_plain"],
-                             "sequential_sync_launcher": sv["sequential_sync_launcher"], "saturation": sv["saturation"]},
+                             "sequential_sync_launcher": sv["sequential_sync_launcher"],
+                             "by_dimension": sv["by_dimension"], "saturation": sv["saturation"]},
         }
     print(json.dumps(line), flush=True)
     return 0
