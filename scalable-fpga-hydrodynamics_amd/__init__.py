@@ -161,6 +161,66 @@ def _is64(t) -> bool:
     raise HydroError(f"float64 or float32 state expected, got {t.dtype}")
 
 
+# The C ABI trusts the sizes it is given, so every tensor is checked here before its
+# pointer is passed on: a wrong shape, dtype or device would make a kernel or a copy
+# read or write past an allocation.  The checks run in the order dtype, shape, device,
+# contiguity, so that shape and dtype errors are reported the same with or without a GPU.
+
+def _state(rhoE, mom, what: str, dim: int | None = None, dtype=None):
+    """Check a (rhoE, momentum) pair in the reference layout: rhoE (NB+2..., 2) and mom
+    (NB+2..., dim) of one float dtype on one device.  Returns (dim, nb, is64)."""
+    is64 = _is64(rhoE)
+    if dtype is not None and rhoE.dtype != dtype:
+        raise HydroError(f"{what}: {dtype} state expected, got {rhoE.dtype}")
+    if mom.dtype != rhoE.dtype:
+        raise HydroError(f"{what}: momentum dtype {mom.dtype} differs from rhoE dtype {rhoE.dtype}")
+    d = rhoE.dim() - 1
+    if d not in (2, 3) or (dim is not None and d != dim) or rhoE.shape[-1] != 2:
+        want = f"{dim}D" if dim is not None else "2D or 3D"
+        raise HydroError(f"{what}: rhoE of shape (NB+2, ..., 2) for {want} expected, got {tuple(rhoE.shape)}")
+    nb = [int(s) - 2 for s in rhoE.shape[:d]]
+    if min(nb) < 1:
+        raise HydroError(f"{what}: every interior size must be >= 1, got {nb}")
+    want_mom = tuple(rhoE.shape[:d]) + (d,)
+    if tuple(mom.shape) != want_mom:
+        raise HydroError(f"{what}: momentum of shape {want_mom} expected, got {tuple(mom.shape)}")
+    if rhoE.device != mom.device:
+        raise HydroError(f"{what}: rhoE on {rhoE.device} and momentum on {mom.device}")
+    return d, nb, is64
+
+
+def _like(ref, what: str, *others) -> None:
+    """Each tensor in `others` has the shape, dtype and device of `ref`."""
+    for t in others:
+        if t.dtype != ref.dtype:
+            raise HydroError(f"{what}: dtype {t.dtype} differs from {ref.dtype}")
+        if tuple(t.shape) != tuple(ref.shape):
+            raise HydroError(f"{what}: shape {tuple(t.shape)} differs from {tuple(ref.shape)}")
+        if t.device != ref.device:
+            raise HydroError(f"{what}: tensor on {t.device}, expected {ref.device}")
+
+
+def _scalar(dt, ref, what: str) -> None:
+    """The Dt_next output: at least one element of the state's dtype on its device."""
+    if dt.dtype != ref.dtype:
+        raise HydroError(f"{what}: Dt_next dtype {dt.dtype} differs from the state's {ref.dtype}")
+    if dt.numel() < 1:
+        raise HydroError(f"{what}: Dt_next must hold one element")
+    if dt.device != ref.device:
+        raise HydroError(f"{what}: Dt_next on {dt.device}, expected {ref.device}")
+
+
+def _nbytes(x) -> int:
+    """Size in bytes of a contiguous torch tensor or numpy array."""
+    if hasattr(x, "data_ptr"):
+        if not x.is_contiguous():
+            raise HydroError("contiguous destination expected")
+        return x.numel() * x.element_size()
+    if not x.flags["C_CONTIGUOUS"]:
+        raise HydroError("contiguous destination expected")
+    return x.nbytes
+
+
 def _sizes(a) -> C.Array:
     return (C.c_size_t * len(a))(*[int(x) for x in a])
 
@@ -186,8 +246,11 @@ def constants(nb, C_cfl: float, dtype: str = "f64") -> dict:
 
 def launcher(rhoE, uv, rhoE_next, uv_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, stream=None):
     """The drop-in 2D launcher (hydro2d/kernel.hpp:63-68): one synchronous step."""
-    nb_x, nb_y = rhoE.shape[0] - 2, rhoE.shape[1] - 2
-    fn = lib().launcher if _is64(rhoE) else lib().launcher_f32
+    _, (nb_x, nb_y), is64 = _state(rhoE, uv, "launcher", dim=2)
+    _like(rhoE, "launcher rhoE_next", rhoE_next)
+    _like(uv, "launcher uv_next", uv_next)
+    _scalar(Dt_next, rhoE, "launcher")
+    fn = lib().launcher if is64 else lib().launcher_f32
     us = fn(_p(rhoE), _p(uv), _p(rhoE_next), _p(uv_next), _p(Dt_next), C_min, gamma, gmo, divgamma, Kc, nb_x, nb_y,
             DtDx, DtDy, _stream(stream))
     if us < 0:
@@ -198,8 +261,11 @@ def launcher(rhoE, uv, rhoE_next, uv_next, Dt_next, C_min, gamma, gmo, divgamma,
 def launcher3d(rhoE, uvw, rhoE_next, uvw_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, DtDz,
                stream=None):
     """The drop-in 3D launcher (hydro3d/kernel.hpp:64-69)."""
-    nb = [s - 2 for s in rhoE.shape[:3]]
-    fn = lib().launcher3d if _is64(rhoE) else lib().launcher3d_f32
+    _, nb, is64 = _state(rhoE, uvw, "launcher3d", dim=3)
+    _like(rhoE, "launcher3d rhoE_next", rhoE_next)
+    _like(uvw, "launcher3d uvw_next", uvw_next)
+    _scalar(Dt_next, rhoE, "launcher3d")
+    fn = lib().launcher3d if is64 else lib().launcher3d_f32
     us = fn(_p(rhoE), _p(uvw), _p(rhoE_next), _p(uvw_next), _p(Dt_next), C_min, gamma, gmo, divgamma, Kc, *nb, DtDx,
             DtDy, DtDz, _stream(stream))
     if us < 0:
@@ -209,27 +275,39 @@ def launcher3d(rhoE, uvw, rhoE_next, uvw_next, Dt_next, C_min, gamma, gmo, divga
 
 def step2d(rhoE, uv, rhoE_next, uv_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, bc_y=BC_REFLECTIVE,
            stream=None):
-    nb_x, nb_y = rhoE.shape[0] - 2, rhoE.shape[1] - 2
+    """One asynchronous FP64 step (hydro_step2d)."""
+    import torch
+
+    _, (nb_x, nb_y), _ = _state(rhoE, uv, "step2d", dim=2, dtype=torch.float64)
+    _like(rhoE, "step2d rhoE_next", rhoE_next)
+    _like(uv, "step2d uv_next", uv_next)
+    _scalar(Dt_next, rhoE, "step2d")
     _check(lib().hydro_step2d(_p(rhoE), _p(uv), _p(rhoE_next), _p(uv_next), _p(Dt_next), C_min, gamma, gmo, divgamma,
                               Kc, nb_x, nb_y, DtDx, DtDy, bc_y, _stream(stream)), "hydro_step2d")
 
 
 def step3d(rhoE, uvw, rhoE_next, uvw_next, Dt_next, C_min, gamma, gmo, divgamma, Kc, DtDx, DtDy, DtDz,
            bc_y=BC_REFLECTIVE, bc_z=BC_REFLECTIVE, stream=None):
-    nb = [s - 2 for s in rhoE.shape[:3]]
+    """One asynchronous FP64 step (hydro_step3d)."""
+    import torch
+
+    _, nb, _ = _state(rhoE, uvw, "step3d", dim=3, dtype=torch.float64)
+    _like(rhoE, "step3d rhoE_next", rhoE_next)
+    _like(uvw, "step3d uvw_next", uvw_next)
+    _scalar(Dt_next, rhoE, "step3d")
     _check(lib().hydro_step3d(_p(rhoE), _p(uvw), _p(rhoE_next), _p(uvw_next), _p(Dt_next), C_min, gamma, gmo, divgamma,
                               Kc, *nb, DtDx, DtDy, DtDz, bc_y, bc_z, _stream(stream)), "hydro_step3d")
 
 
 def ghost_fill2d(rhoE, uv, bc_x=BC_PERIODIC, bc_y=BC_REFLECTIVE, stream=None):
-    nb_x, nb_y = rhoE.shape[0] - 2, rhoE.shape[1] - 2
-    fn = lib().hydro_ghost_fill2d if _is64(rhoE) else lib().hydro_ghost_fill2d_f32
+    _, (nb_x, nb_y), is64 = _state(rhoE, uv, "ghost_fill2d", dim=2)
+    fn = lib().hydro_ghost_fill2d if is64 else lib().hydro_ghost_fill2d_f32
     _check(fn(_p(rhoE), _p(uv), nb_x, nb_y, bc_x, bc_y, _stream(stream)), "hydro_ghost_fill2d")
 
 
 def ghost_fill3d(rhoE, uvw, bc_x=BC_PERIODIC, bc_y=BC_REFLECTIVE, bc_z=BC_REFLECTIVE, stream=None):
-    nb = [s - 2 for s in rhoE.shape[:3]]
-    fn = lib().hydro_ghost_fill3d if _is64(rhoE) else lib().hydro_ghost_fill3d_f32
+    _, nb, is64 = _state(rhoE, uvw, "ghost_fill3d", dim=3)
+    fn = lib().hydro_ghost_fill3d if is64 else lib().hydro_ghost_fill3d_f32
     _check(fn(_p(rhoE), _p(uvw), *nb, bc_x, bc_y, bc_z, _stream(stream)), "hydro_ghost_fill3d")
 
 
@@ -238,10 +316,10 @@ def ic(rhoE, mom, C_min, NB=None, base=None, ic_mode=0, radius_frac=0.0, rho_in=
     """Blast IC on the device (hydro2d/main.cxx:280-338); returns the initial Dt."""
     import numpy as np
 
-    dim = rhoE.dim() - 1
-    nb = [s - 2 for s in rhoE.shape[:dim]]
+    dim, nb, is64 = _state(rhoE, mom, "ic")
     NB = nb if NB is None else NB
-    is64 = _is64(rhoE)
+    if len(NB) != dim or (base is not None and len(base) != dim):
+        raise HydroError(f"ic: NB and base must have {dim} entries")
     out = (C.c_double if is64 else C.c_float)()
     fn = lib().hydro_ic if is64 else lib().hydro_ic_f32
     t = np.float64 if is64 else np.float32
@@ -251,9 +329,17 @@ def ic(rhoE, mom, C_min, NB=None, base=None, ic_mode=0, radius_frac=0.0, rho_in=
     return t(out.value)
 
 
-def total_mass(rhoE, stream=None) -> float:
+def _rhoE_only(rhoE, what: str):
+    """Check rhoE alone (the mass and energy sums read no momentum)."""
+    is64 = _is64(rhoE)
     dim = rhoE.dim() - 1
-    nb = [s - 2 for s in rhoE.shape[:dim]]
+    if dim not in (2, 3) or rhoE.shape[-1] != 2 or min(rhoE.shape[:dim]) < 3:
+        raise HydroError(f"{what}: rhoE of shape (NB+2, ..., 2) expected, got {tuple(rhoE.shape)}")
+    return dim, [int(s) - 2 for s in rhoE.shape[:dim]], is64
+
+
+def total_mass(rhoE, stream=None) -> float:
+    dim, nb, _ = _rhoE_only(rhoE, "total_mass")
     m = C.c_double()
     _check(lib().hydro_total_mass(_p(rhoE), nb[0], nb[1], nb[2] if dim == 3 else 0, 64 if _is64(rhoE) else 32,
                                   C.byref(m), _stream(stream)), "hydro_total_mass")
@@ -262,8 +348,7 @@ def total_mass(rhoE, stream=None) -> float:
 
 def total_energy(rhoE, stream=None) -> float:
     """Sum of interior E (compensated, on the device)."""
-    dim = rhoE.dim() - 1
-    nb = [s - 2 for s in rhoE.shape[:dim]]
+    dim, nb, _ = _rhoE_only(rhoE, "total_energy")
     m = C.c_double()
     _check(lib().hydro_total_energy(_p(rhoE), nb[0], nb[1], nb[2] if dim == 3 else 0, 64 if _is64(rhoE) else 32,
                                     C.byref(m), _stream(stream)), "hydro_total_energy")
@@ -277,6 +362,10 @@ class Run:
 
     def __init__(self, nb, D, C_min, dt0, max_t, max_it, precision=64, bc=None):
         self.dim = len(nb)
+        self.nb = [int(n) for n in nb]
+        self.precision = int(precision)
+        if len(D) != self.dim:
+            raise HydroError(f"Run: {self.dim} spacings expected, got {len(D)}")
         bcs = list(bc) if bc is not None else [BC_PERIODIC] + [BC_REFLECTIVE] * (self.dim - 1)
         self._bc = (C.c_int * 3)(*(bcs + [0] * (3 - len(bcs))))
         self.max_it = int(max_it)
@@ -286,7 +375,21 @@ class Run:
             raise HydroError(f"hydro_run_create failed: {last_error()}")
         self._h = C.c_void_p(h)
 
+    def _check_buffers(self, a, b) -> None:
+        """Both buffer pairs match the run: its dimension, interior sizes and precision
+        (float32 state for FP32, float64 for FP64 and FP32_FLUX)."""
+        import torch
+
+        dtype = torch.float32 if self.precision == FP32 else torch.float64
+        for name, pair in (("a", a), ("b", b)):
+            _, nb, _ = _state(pair[0], pair[1], f"Run.advance buffer {name}", dim=self.dim, dtype=dtype)
+            if nb != self.nb:
+                raise HydroError(f"Run.advance buffer {name}: interior {nb}, the run has {self.nb}")
+        _like(a[0], "Run.advance buffer b rhoE", b[0])
+        _like(a[1], "Run.advance buffer b momentum", b[1])
+
     def advance(self, a, b, n_steps, stream=None):
+        self._check_buffers(a, b)
         _check(lib().hydro_run_advance(self._h, _p(a[0]), _p(a[1]), _p(b[0]), _p(b[1]), int(n_steps), _stream(stream)),
                "hydro_run_advance")
 
@@ -341,6 +444,7 @@ class Batch:
             raise HydroError(f"hydro_batch_create failed: {last_error()}")
         self._h = C.c_void_p(h)
         self.dim = dim
+        self.precision = int(precision)
         self.capacity = capacity
         self._bufs = {}
 
@@ -355,8 +459,21 @@ class Batch:
 
         torch.cuda.ExternalStream(self.stream).wait_stream(torch.cuda.current_stream())
 
+    def _check_buffers(self, a, b, what: str):
+        """Both buffer pairs hold float64 state (FP64 and FP32_FLUX keep FP64 state) of
+        the batch's dimension and of equal sizes; returns the interior sizes."""
+        import torch
+
+        _, nb, _ = _state(a[0], a[1], f"{what} buffer a", dim=self.dim, dtype=torch.float64)
+        _state(b[0], b[1], f"{what} buffer b", dim=self.dim, dtype=torch.float64)
+        _like(a[0], f"{what} buffer b rhoE", b[0])
+        _like(a[1], f"{what} buffer b momentum", b[1])
+        return tuple(nb)
+
     def add(self, a, b, D, C_min, dt0, max_t, max_it) -> int:
-        nb = tuple(n - 2 for n in a[0].shape[: self.dim])
+        nb = self._check_buffers(a, b, "Batch.add")
+        if len(D) != self.dim:
+            raise HydroError(f"Batch.add: {self.dim} spacings expected, got {len(D)}")
         self._after_current_stream()
         slot = lib().hydro_batch_add(self._h, _sizes(nb), (C.c_double * 3)(*[float(x) for x in D] + [0.0] * (3 - len(D))),
                                      float(C_min), float(dt0), float(max_t), int(max_it), _p(a[0]), _p(a[1]), _p(b[0]),
@@ -368,7 +485,7 @@ class Batch:
 
     def add_blast(self, a, b, C, radius_frac, p_in, p_out=0.1, rho_in=1.0, rho_out=1.2, max_t=1e30,
                   max_it=100) -> int:
-        nb = tuple(n - 2 for n in a[0].shape[: self.dim])
+        nb = self._check_buffers(a, b, "Batch.add_blast")
         self._after_current_stream()
         slot = lib().hydro_batch_add_blast(self._h, _sizes(nb), float(C), float(radius_frac), float(rho_in),
                                            float(rho_out), float(p_in), float(p_out), float(max_t), int(max_it),
@@ -453,6 +570,7 @@ class Server:
         if not h:
             raise HydroError(f"hydro_server_create failed: {last_error()}")
         self._h = C.c_void_p(h)
+        self._nb = {}
 
     def now(self) -> float:
         return lib().hydro_server_now(self._h)
@@ -466,6 +584,7 @@ class Server:
                                         int(max_it), 1 if keep else 0, float(arrival))
         if rid < 0:
             raise HydroError(f"hydro_server_submit failed ({rid}): {last_error()}")
+        self._nb[rid] = [int(n) for n in nb]
         return rid
 
     def tick(self) -> int:
@@ -486,11 +605,29 @@ class Server:
                 "t": ts.value, "rc": rc}
 
     def result(self, rid: int, rhoE, mom) -> None:
-        """Copy a kept result into (rhoE, mom) (device or host tensors / numpy arrays)."""
+        """Copy a kept result into (rhoE, mom) (device or host tensors / numpy arrays),
+        each at least as large as the request's state with ghosts: 2 (rhoE) and dim
+        (momentum) float64 values per cell."""
         def ptr(x):
             return C.c_void_p(x.data_ptr()) if hasattr(x, "data_ptr") else x.ctypes.data_as(C.c_void_p)
 
-        _check(lib().hydro_server_result(self._h, int(rid), ptr(rhoE), ptr(mom)), "hydro_server_result")
+        nb = self._nb.get(int(rid))
+        if nb is None:
+            raise HydroError(f"Server.result: unknown request {rid}")
+        cells = 1
+        for n in nb:
+            cells *= n + 2
+        for name, dst, per_cell in (("rhoE", rhoE, 2), ("momentum", mom, len(nb))):
+            if dst is None:
+                continue
+            if str(dst.dtype) not in ("torch.float64", "float64"):
+                raise HydroError(f"Server.result: float64 {name} destination expected, got {dst.dtype}")
+            if _nbytes(dst) < cells * per_cell * 8:
+                raise HydroError(f"Server.result: {name} destination holds {_nbytes(dst)} bytes, the result "
+                                 f"needs {cells * per_cell * 8}")
+
+        _check(lib().hydro_server_result(self._h, int(rid), ptr(rhoE) if rhoE is not None else None,
+                                         ptr(mom) if mom is not None else None), "hydro_server_result")
 
     def release(self, rid: int) -> None:
         _check(lib().hydro_server_release(self._h, int(rid)), "hydro_server_release")
@@ -519,6 +656,8 @@ class Server:
 def stream_copy(dst, src, stream=None) -> None:
     """Plain device copy (achievable-bandwidth reference)."""
     nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() < nbytes or dst.device != src.device:
+        raise HydroError("stream_copy: dst must be at least as large as src, on the same device")
     _check(lib().hydro_stream_copy(_p(dst), _p(src), nbytes, _stream(stream)), "hydro_stream_copy")
 
 

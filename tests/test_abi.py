@@ -100,3 +100,25 @@ def test_missing_library_fails_loudly(tmp_path):
     env = {"HYDRO_LIB_PATH": str(tmp_path / "missing.so"), "PATH": "/usr/bin:/bin"}
     p = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0 and "is not built" in p.stdout, (p.stdout, p.stderr)
+
+
+@pytest.mark.gpu
+def test_trajectory_range_cannot_wrap(gpu):
+    """from + count must not wrap in size_t: a range starting at SIZE_MAX is rejected
+    instead of passing the check and copying from far outside the record."""
+    L = gpu.lib()
+    sizes = (C.c_size_t * 3)(16, 16, 0)
+    D = (C.c_double * 3)(1 / 17, 1 / 17, 0.0)
+    h = L.hydro_run_create(2, sizes, D, 0.5 / 17, 1e-3, 1.0, 10, 64, None)
+    assert h
+    try:
+        t = (C.c_double * 4)()
+        size_max = (1 << 64) - 1
+        assert L.hydro_run_trajectory(h, size_max, 2, t, None, None) == -1
+        assert "outside" in gpu.last_error()
+        assert L.hydro_run_trajectory(h, 9, size_max, t, None, None) == -1
+        assert L.hydro_run_trajectory(h, 11, 0, t, None, None) == -1
+        assert L.hydro_run_trajectory(h, 10, 0, t, None, None) == 0
+        assert L.hydro_run_trajectory(h, 6, 4, t, None, None) == 0
+    finally:
+        L.hydro_run_destroy(h)
