@@ -11,6 +11,7 @@ Line formats (hydro2d/main.cxx:673-674, 809-811, 273-274):
   "Initial Dt : %.2e"
   "[it=%4lu] hydro_fvm compute time t=%.4e (Dt=%.3e) : ..."
   "[it=%4lu] t=%.4e Total density : %.7e, change in mass: %.1e ..."
+  "Mass change : %.2e"                                  (hydro2d/main.cxx:887)
 """
 import json
 import pathlib
@@ -36,7 +37,7 @@ def parse(path: pathlib.Path, dim: int) -> dict:
                 cfg[key] = line.split(":")[-1].strip()
     traj = []
     masses = []
-    init_mass = init_dt = None
+    init_mass = init_dt = mass_change = None
     for line in text:
         m = IT_RE.match(line)
         if m:
@@ -50,6 +51,8 @@ def parse(path: pathlib.Path, dim: int) -> dict:
             init_mass = line.split(":")[-1].strip()
         elif line.startswith("Initial Dt"):
             init_dt = line.split(":")[-1].strip()
+        elif line.startswith("Mass change :"):
+            mass_change = line.split(":")[-1].strip()
     for i, row in enumerate(traj):
         assert row[0] == i, (path, i, row)
     return {
@@ -63,6 +66,7 @@ def parse(path: pathlib.Path, dim: int) -> dict:
         "initial_dt": init_dt,
         "trajectory": traj,
         "mass_lines": masses,
+        "mass_change": mass_change,
     }
 
 
@@ -77,7 +81,7 @@ def main() -> int:
             runs[key] = parse(p, dim)
     OUT.write_text(json.dumps(runs, separators=(",", ":")))
     for k, v in runs.items():
-        print(k, v["nb"], v["ranks"], v["initial_mass"], v["initial_dt"], len(v["trajectory"]), v["mass_lines"][-1])
+        print(k, v["nb"], v["ranks"], v["initial_mass"], v["initial_dt"], len(v["trajectory"]), v["mass_lines"][-1], v["mass_change"])
     return 0
 
 

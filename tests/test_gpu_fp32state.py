@@ -2,8 +2,10 @@
 every state value and constant in FP32, bit for bit against the oracle's FP32 build.
 
 The constants are computed in FP32 as the reference computes them (gamma - ONE in
-float is not float(1.4 - 1.0)), so they come from the oracle's constants() in float32
-and reach the launchers as exact floats.
+float is not float(1.4 - 1.0)): the product side takes them from the package's own
+constants(..., "f32"), the oracle side from the oracle's, and
+tests/test_oracle_pins.py::test_fp32_constants_are_float_arithmetic pins both to float
+arithmetic formed independently in numpy.
 """
 import numpy as np
 import pytest
@@ -17,8 +19,8 @@ F32 = np.float32
 SENTINEL = 12345.0
 
 
-def _k(oracle, nb, C):
-    return oracle.constants(nb, C, F32)
+def _k(gpu, nb, C):
+    return gpu.constants(nb, C, "f32")
 
 
 def _launch(gpu, dim, d, n, dtn, k, r):
@@ -30,7 +32,7 @@ def _launch(gpu, dim, d, n, dtn, k, r):
 @pytest.mark.parametrize("nb", [(5, 3), (33, 62), (20, 63), (130, 125)])
 def test_launcher_f32_random_states_2d(gpu, oracle, nb):
     rhoE, uv = random_state_2d(*nb, seed=7 * nb[0] + nb[1], dtype=F32)
-    k = _k(oracle, nb, 0.5)
+    k = _k(gpu, nb, 0.5)
     r = (F32(0.02), F32(0.02))
     want_r, want_u, want_dt = oracle.launcher2d(rhoE, uv, *r, k["C_min"])
     assert np.isfinite(want_dt)
@@ -46,7 +48,7 @@ def test_launcher_f32_random_states_2d(gpu, oracle, nb):
 @pytest.mark.parametrize("nb", [(4, 5, 7), (10, 6, 62), (9, 7, 63), (40, 8, 125), (33, 20, 30)])
 def test_launcher_f32_random_states_3d(gpu, oracle, nb):
     rhoE, uvw = random_state_3d(*nb, seed=nb[0] * 10000 + nb[1] * 100 + nb[2], dtype=F32)
-    k = _k(oracle, nb, 0.4)
+    k = _k(gpu, nb, 0.4)
     r = (F32(0.015), F32(0.015), F32(0.015))
     want_r, want_u, want_dt = oracle.launcher3d(rhoE, uvw, *r, k["C_min"])
     assert np.isfinite(want_dt)
@@ -65,10 +67,10 @@ def test_launcher_f32_blast_sequence(gpu, oracle, dim, nb, steps):
     DATATYPE = float: Dt / DX is formed in float."""
     if dim == 2:
         rhoE, mom, dt = oracle.ic2d(*nb, dtype=F32)
-        k = _k(oracle, nb, 0.5)
+        k = _k(gpu, nb, 0.5)
     else:
         rhoE, mom, dt = oracle.ic3d(*nb, dtype=F32)
-        k = _k(oracle, nb, 0.4)
+        k = _k(gpu, nb, 0.4)
     d = [to_dev(rhoE), to_dev(mom)]
     n = [torch.zeros_like(d[0]), torch.zeros_like(d[1])]
     dtn = torch.zeros(1, dtype=torch.float32, device="cuda")
@@ -104,10 +106,10 @@ def test_resident_run_f32(gpu, oracle, dim, nb, bc, steps):
     step's Dt and t, and the final fields."""
     if dim == 2:
         rhoE, mom, dt0 = oracle.ic2d(*nb, dtype=F32)
-        k = _k(oracle, nb, 0.5)
+        k = _k(gpu, nb, 0.5)
     else:
         rhoE, mom, dt0 = oracle.ic3d(*nb, dtype=F32)
-        k = _k(oracle, nb, 0.4)
+        k = _k(gpu, nb, 0.4)
     a = [to_dev(rhoE), to_dev(mom)]
     b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
     run = gpu.Run(list(nb), k["D"], k["C_min"], dt0, 2.0, steps, 32, list(bc))

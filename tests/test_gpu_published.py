@@ -4,11 +4,18 @@ For each run in tests/golden/res_logs.json (hydro2d/reference_run/*.res,
 hydro3d/reference_run/*.res) the device-resident loop reproduces: the initial mass and
 Dt, every per-iteration (t, Dt) pair the log prints (t to 5 and Dt to 4 significant
 digits) within the first `exact` iterations -- all 2000 of them for five of the six
-runs -- and later ones to 2e-3 relative (in the 170^3 run the published FPGA arithmetic,
-which is not strict IEEE, drifts by one unit in the last printed Dt digit after
-iteration 1232), the stopping iteration and the final t and mass.  The
+runs -- and later ones to 2e-3 relative (in the 170^3 run the published FPGA arithmetic
+drifts by one unit in the last printed Dt digit after iteration 1232), the stopping
+iteration, the final t and the final total density to its 8 printed digits.  The
 two-FPGA runs are one resident domain here; with exact ghost exchange and a global MIN
 on Dt they are the same computation (hydro2d/main.cxx:381-424, 781).
+
+Total mass is computed as the reference prints it: a compensated long double sum of
+interior rho per rank, rounded to double, then the ranks' doubles summed
+(compute_total_mass, hydro2d/main.cxx:237-265, hydro3d/main.cxx:249-281); a two-rank
+run is the X split into two slabs of NB_X/2 rows (hydro2d/main.cxx:621-636).  The
+printed change in mass |m0 - m1| / cells (hydro2d/main.cxx:267-278, 887) is then
+compared with the log's %.1e and %.2e values in test_published_mass_change.
 """
 import math
 
@@ -28,6 +35,9 @@ RUNS = [
     ("2d_40000x20000x2FPGAS", 0.5, 2000),
 ]
 
+# name -> (initial mass, final mass, interior cells), filled by test_published_run
+_MASSES = {}
+
 
 def _close_print(value: float, printed: str, digits: int) -> bool:
     if f"{value:.{digits - 1}e}" == printed:
@@ -35,6 +45,25 @@ def _close_print(value: float, printed: str, digits: int) -> bool:
     p = float(printed)
     unit = 10.0 ** (math.floor(math.log10(abs(p))) - (digits - 1))
     return abs(value - p) <= 0.5000001 * unit
+
+
+def reference_mass(oracle, rhoE_dev, ranks: int) -> float:
+    """compute_total_mass of the state on the device: per X slab (rank) a compensated
+    long double sum in i-major order, rounded to double; the slabs' doubles summed in
+    rank order as MPI_Allreduce(SUM) does for two ranks."""
+    torch.cuda.synchronize()
+    host = rhoE_dev.cpu().numpy()
+    nb_x = host.shape[0] - 2
+    sub = nb_x // ranks
+    fn = oracle.mass3d if host.ndim == 4 else oracle.mass2d
+    total = 0.0
+    for r in range(ranks):
+        lo = r * sub
+        rows = sub + (nb_x % ranks if r == ranks - 1 else 0)
+        # planes lo .. lo+rows+1: the slab's rows with one neighbour plane on each side,
+        # which the sum skips as it skips ghost planes
+        total += fn(np.ascontiguousarray(host[lo:lo + rows + 2]))
+    return total
 
 
 @pytest.mark.parametrize("name,C,exact", RUNS)
@@ -48,7 +77,7 @@ def test_published_run(gpu, oracle, name, C, exact):
          torch.zeros(shape + (len(nb),), dtype=torch.float64, device="cuda")]
     b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
     dt0 = gpu.ic(a[0], a[1], k["C_min"])
-    m0 = gpu.total_mass(a[0])
+    m0 = reference_mass(oracle, a[0], r["ranks"])
     assert f"{m0:.7e}" == r["initial_mass"]
     assert f"{dt0:.2e}" == r["initial_dt"]
     run = gpu.Run(nb, k["D"], k["C_min"], dt0, r["max_t"], r["max_it"])
@@ -70,9 +99,44 @@ def test_published_run(gpu, oracle, name, C, exact):
     assert final[0] == st["it"]
     assert f"{st['t']:.4e}" == final[1]
     cur = a if st["current"] == 0 else b
-    m1 = gpu.total_mass(cur[0])
+    m1 = reference_mass(oracle, cur[0], r["ranks"])
+    cells = float(np.prod(nb))
+    _MASSES[name] = (m0, m1, cells)
+    print(f"[published] {name}: m0 {m0!r} m1 {m1!r} change in mass {abs(m0 - m1) / cells:.3e} "
+          f"(published {final[3]} / {r['mass_change']})", flush=True)
     assert f"{m1:.7e}" == final[2]
-    assert abs(m1 - m0) / np.prod(nb) < 1e-13
+    # the strict scheme conserves mass to round-off: the totals differ by a few units
+    # in the last place (0 to 2 measured), against 40 to 8e4 in the published logs
+    assert abs(m1 - m0) <= 4 * r["ranks"] * math.ulp(m0)
     run.close()
     del a, b, cur
     torch.cuda.empty_cache()
+
+
+# The published runs' printed change in mass cannot be reproduced by the strict scheme,
+# which conserves mass to the rounding of the double total: 0.0e+00 on both 2D runs and
+# 0 to 3.8e-16 per cell (0 to 2 units in the last place of the total) on the 3D runs
+# here, bit-identical to the oracle.  The logs print 5.96e-15 and 2.53e-15 (2D: 40 and
+# 34 units in the last place) and 9.89e-12, 1.27e-12, 1.50e-11, 5.26e-13 (3D: 2.0e3 to
+# 7.9e4 units).  Interior faces are one value for both their cells and walls carry
+# exactly zero mass flux, so the FPGA build's arithmetic differs from the operation
+# sequence of hydro2d/kernel.cxx and hydro3d/kernel.cxx under IEEE rounding.  The
+# oracle run under other FP models (oracle/fp_model_study.c) matches neither dimension:
+# round toward zero gives 3.8e-13 on 170^3 (26x below the log) and a 2D loss that
+# extrapolates to about 2.7e-14 at 20000^2 (4.5x above it); FMA contraction conserves
+# as the strict build does.
+_PUBLISHED_FP_MODEL = pytest.mark.xfail(
+    reason="the published FPGA runs lose mass (40 to 8e4 ulps of the total); the strict scheme conserves it "
+           "to the total's rounding, and no IEEE rounding or contraction model reproduces the logs",
+    strict=False)
+
+
+@pytest.mark.parametrize("name", [pytest.param(n, marks=_PUBLISHED_FP_MODEL) for n, _, _ in RUNS])
+def test_published_mass_change(oracle, name):
+    """The printed change in mass (%.1e on the mass line, %.2e in the summary)."""
+    if name not in _MASSES:
+        pytest.skip("the run itself did not complete")
+    r = oracle.load_res_logs()[name]
+    m0, m1, cells = _MASSES[name]
+    change = abs(m0 - m1) / cells
+    assert (f"{change:.1e}", f"{change:.2e}") == (r["mass_lines"][-1][3], r["mass_change"])
