@@ -51,11 +51,22 @@ extern "C" {
  * (each thread, for hipStreamPerThread) has its own 512-byte Dt reduction state, bound
  * to it at its first step and kept until the process ends.  The states come from a pool
  * of 256 per device in the library's device image, so the launchers allocate nothing;
- * a device's 257th and later streams get a hipMalloc'd state.  Steps on one stream run
- * in stream order, as any HIP work does.  A new stream that reuses the handle of a
- * destroyed one takes over its state, so let a stream's steps finish before destroying
- * it.
+ * while all 256 are bound, a further stream gets a hipMalloc'd state.  Steps on one
+ * stream run in stream order, as any HIP work does.  hydro_stream_release unbinds a
+ * stream's state (before the stream is destroyed, when its handle may be reused or
+ * many streams come and go); without it a state stays bound until the process ends,
+ * and a new stream that reuses the handle of a destroyed one takes over its state.
  * ------------------------------------------------------------------------------- */
+
+/* Release the Dt reduction state bound to `stream` on the current device (for
+ * hipStreamPerThread: the calling thread's).  Waits for the stream, then returns the
+ * state to the device's pool, or frees it if it was allocated.  No effect if no step ran
+ * on the stream.  No counterpart in the reference (its launcher owns one queue). */
+int hydro_stream_release(void* stream);
+
+/* Number of streams with a bound state on the current device, and of the device's 256
+ * pooled states that are free (diagnostics for hydro_stream_release). */
+int hydro_stream_states(int* bound, int* pooled_free);
 
 /* Replaces `launcher` of hydro2d/kernel.hpp:63-68 (defined at hydro2d/kernel.cxx:331-367).
  * Same parameter order and meaning; sycl::queue becomes a hipStream_t and the
@@ -339,6 +350,11 @@ int hydro_total_energy(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z
  * kernel: the achievable-bandwidth reference for the step kernels, which read and write
  * the state once.  No counterpart in the reference (measurement only). */
 int hydro_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
+
+/* The same copy with non-temporal loads and stores: the fastest read-once / write-once
+ * stream of this library, quoted as the achievable ceiling of the step kernels.  No
+ * counterpart in the reference (measurement only). */
+int hydro_stream_copy_nt(void* dst, const void* src, size_t bytes, void* stream);
 
 /* Read-only and write-only streams over `bytes` (a multiple of 16): the achievable
  * bandwidth of each direction alone.  hydro_stream_read XOR-folds every 64-bit word of

@@ -87,6 +87,9 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_total_energy.argtypes = [vp, sz, sz, sz, i32, vp, vp]
     L.hydro_stream_copy.argtypes = [vp, vp, sz, vp]
+    L.hydro_stream_copy_nt.argtypes = [vp, vp, sz, vp]
+    L.hydro_stream_release.argtypes = [vp]
+    L.hydro_stream_states.argtypes = [vp, vp]
     L.hydro_stream_read.argtypes = [vp, sz, vp, vp]
     L.hydro_stream_fill.argtypes = [vp, sz, C.c_ulonglong, vp]
     L.hydro_batch_create.argtypes = [i32, i32, sz]
@@ -653,12 +656,14 @@ class Server:
             pass
 
 
-def stream_copy(dst, src, stream=None) -> None:
-    """Plain device copy (achievable-bandwidth reference)."""
+def stream_copy(dst, src, stream=None, nontemporal: bool = False) -> None:
+    """Plain device copy (achievable-bandwidth reference); nontemporal: with
+    non-temporal loads and stores."""
     nbytes = src.numel() * src.element_size()
     if dst.numel() * dst.element_size() < nbytes or dst.device != src.device:
         raise HydroError("stream_copy: dst must be at least as large as src, on the same device")
-    _check(lib().hydro_stream_copy(_p(dst), _p(src), nbytes, _stream(stream)), "hydro_stream_copy")
+    fn = lib().hydro_stream_copy_nt if nontemporal else lib().hydro_stream_copy
+    _check(fn(_p(dst), _p(src), nbytes, _stream(stream)), "hydro_stream_copy")
 
 
 def stream_read(src, slots, stream=None) -> None:
@@ -674,6 +679,18 @@ def stream_fill(dst, pattern: int, stream=None) -> None:
     """Write-only stream: the 64-bit `pattern` to every word of dst."""
     nbytes = dst.numel() * dst.element_size()
     _check(lib().hydro_stream_fill(_p(dst), nbytes, int(pattern) & (2**64 - 1), _stream(stream)), "hydro_stream_fill")
+
+
+def stream_release(stream=None) -> None:
+    """Unbind the plain launchers' Dt reduction state from `stream` (waits for it)."""
+    _check(lib().hydro_stream_release(_stream(stream)), "hydro_stream_release")
+
+
+def stream_states():
+    """(streams with a bound reduction state, free pooled states) on the current device."""
+    bound, free = C.c_int(), C.c_int()
+    _check(lib().hydro_stream_states(C.byref(bound), C.byref(free)), "hydro_stream_states")
+    return bound.value, free.value
 
 
 def tune_set(key: str, value: int) -> None:

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <vector>
 
 #include "../../include/hydro_hip.h"
 #include "hydro_kernels.hpp"
@@ -68,46 +69,69 @@ __device__ StepReduce g_plain_red[kPlainStates];
 // stream), and per thread for the per-thread default stream, whose handle every thread
 // shares.  Steps on one stream run in order and can share a state; steps on different
 // streams may overlap on the device and must not (their atomic maxima would mix, and one
-// finalize would clear the other's slots).  A state is taken from g_plain_red, or
-// allocated (sizeof(StepReduce) bytes) once a device's pool is used up, and is kept until
-// the process ends.  It is cleared here on s, and by the finalize kernel after every
-// step; the caller enqueues its step on s right after.
+// finalize would clear the other's slots).  A state is taken from the device's free
+// list, then from g_plain_red, or allocated (sizeof(StepReduce) bytes) once a device's
+// pool is used up.  It is bound to its stream until hydro_stream_release (which returns
+// it to the free list, or frees an allocated one) or the end of the process.  It is
+// cleared here on s, and by the finalize kernel after every step; the caller enqueues
+// its step on s right after.
+namespace {
+
+struct StateKey {
+    int dev;
+    hipStream_t s;
+    std::thread::id t;
+    bool operator<(const StateKey& o) const
+    {
+        if (dev != o.dev) return dev < o.dev;
+        if (s != o.s) return std::less<hipStream_t>()(s, o.s);
+        return t < o.t;
+    }
+};
+
+struct StatePool {
+    StepReduce* base = nullptr;     // g_plain_red on this device
+    int used = 0;                   // entries of g_plain_red handed out so far
+    std::vector<StepReduce*> free;  // released entries of g_plain_red
+};
+
+std::mutex g_state_mu;
+std::map<StateKey, StepReduce*> g_states;
+std::map<int, StatePool> g_pools;
+
+StateKey state_key(int dev, hipStream_t s)
+{
+    return StateKey{ dev, s, s == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id() };
+}
+
+bool in_pool(const StatePool& pool, const StepReduce* p)
+{
+    return pool.base && p >= pool.base && p < pool.base + kPlainStates;
+}
+
+}  // namespace
+
 StepReduce* plain_reduce_state(hipStream_t s)
 {
-    struct Key {
-        int dev;
-        hipStream_t s;
-        std::thread::id t;
-        bool operator<(const Key& o) const
-        {
-            if (dev != o.dev) return dev < o.dev;
-            if (s != o.s) return std::less<hipStream_t>()(s, o.s);
-            return t < o.t;
-        }
-    };
-    struct Pool {
-        StepReduce* base = nullptr;
-        int used = 0;
-    };
-    static std::mutex mu;
-    static std::map<Key, StepReduce*> states;
-    static std::map<int, Pool> pools;
     const int dev = current_device();
     if (dev < 0) return nullptr;
-    const Key key{ dev, s, s == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id() };
-    std::lock_guard<std::mutex> lk(mu);
-    const auto it = states.find(key);
-    if (it != states.end()) return it->second;
+    const StateKey key = state_key(dev, s);
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    const auto it = g_states.find(key);
+    if (it != g_states.end()) return it->second;
     StepReduce* p = nullptr;
     bool allocated = false;
     try {
-        Pool& pool = pools[dev];
+        StatePool& pool = g_pools[dev];
         if (!pool.base) {
             void* a = nullptr;
             HYDRO_HIP_OR(hipGetSymbolAddress(&a, HIP_SYMBOL(g_plain_red)), nullptr);
             pool.base = static_cast<StepReduce*>(a);
         }
-        if (pool.used < kPlainStates) {
+        if (!pool.free.empty()) {
+            p = pool.free.back();
+            pool.free.pop_back();
+        } else if (pool.used < kPlainStates) {
             p = pool.base + pool.used++;
         } else {
             HYDRO_HIP_OR(hipMalloc(reinterpret_cast<void**>(&p), sizeof(StepReduce)), nullptr);
@@ -120,9 +144,10 @@ StepReduce* plain_reduce_state(hipStream_t s)
         if (e != hipSuccess) {
             set_error("reduction state: clear kernel launch failed: %s", hipGetErrorString(e));
             if (allocated) (void)hipFree(p);
+            else pool.free.push_back(p);
             return nullptr;
         }
-        states.emplace(key, p);
+        g_states.emplace(key, p);
     } catch (...) {
         if (allocated) (void)hipFree(p);
         set_error("reduction state: out of host memory");
@@ -132,6 +157,51 @@ StepReduce* plain_reduce_state(hipStream_t s)
 }
 
 }  // namespace hydro
+
+extern "C" int hydro_stream_release(void* stream)
+try {
+    using namespace hydro;
+    clear_error();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int dev = current_device();
+    if (dev < 0) return HYDRO_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    const auto it = g_states.find(state_key(dev, s));
+    if (it == g_states.end()) return 0;  // no step ran on this stream
+    // The stream's steps must be done with the state before another stream takes it.
+    HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
+    StepReduce* p = it->second;
+    g_states.erase(it);
+    StatePool& pool = g_pools[dev];
+    if (in_pool(pool, p)) {
+        pool.free.push_back(p);
+    } else {
+        HYDRO_HIP_OR(hipFree(p), HYDRO_ERR_HIP);
+    }
+    return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
+}
+
+extern "C" int hydro_stream_states(int* bound, int* pooled_free)
+{
+    using namespace hydro;
+    const int dev = current_device();
+    if (dev < 0) return HYDRO_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_state_mu);
+    int n = 0;
+    for (const auto& kv : g_states)
+        if (kv.first.dev == dev) ++n;
+    if (bound) *bound = n;
+    if (pooled_free) {
+        const auto it = g_pools.find(dev);
+        *pooled_free = it == g_pools.end()
+                           ? kPlainStates
+                           : kPlainStates - it->second.used + static_cast<int>(it->second.free.size());
+    }
+    return 0;
+}
 
 extern "C" const char* hydro_last_error(void) { return g_last_error; }
 

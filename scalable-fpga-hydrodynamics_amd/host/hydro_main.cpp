@@ -157,11 +157,12 @@ struct HostCopy {
     T* re = nullptr;
     T* mo = nullptr;
     Writer writer;  // declared after the buffers: destroyed (joined) before they are freed
+    // n_mo = 0: the (rho, E) array only, for the mass sum.
     void alloc(size_t n_re, size_t n_mo)
     {
-        if (re) return;
-        if (hipHostMalloc(reinterpret_cast<void**>(&re), n_re * sizeof(T), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc(reinterpret_cast<void**>(&mo), n_mo * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+        if ((!re && hipHostMalloc(reinterpret_cast<void**>(&re), n_re * sizeof(T), hipHostMallocDefault) != hipSuccess) ||
+            (n_mo && !mo &&
+             hipHostMalloc(reinterpret_cast<void**>(&mo), n_mo * sizeof(T), hipHostMallocDefault) != hipSuccess)) {
             fprintf(stderr, "Error: cannot allocate %.2f GB of pinned host memory\n",
                     static_cast<double>((n_re + n_mo) * sizeof(T)) / 1e9);
             exit(1);
@@ -325,11 +326,18 @@ int run(const Options& o, const char* argv0)
         die_hip(hipMemcpy(host.re, d_re[cur], n_re * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
         die_hip(hipMemcpy(host.mo, d_mo[cur], n_mo * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     };
-    auto device_mass = [&](int cur) {
-        double m = 0;
-        die_hydro(hydro_total_mass(d_re[cur], nb[0], nb[1], kDim == 3 ? nb[2] : 0, sizeof(T) == 8 ? 64 : 32, &m, s),
-                  "hydro_total_mass");
-        return m;
+    // The mass the reference prints, compute_total_mass (hydro2d/main.cxx:237-265): the
+    // host compensated long double sum of interior rho, from the (rho, E) array alone
+    // when no CSV needs the momenta (--no-output).
+    auto host_mass = [&](int cur) {
+        if (!o.no_output) {
+            fetch(cur);
+        } else {
+            host.writer.join();
+            host.alloc(n_re, 0);
+            die_hip(hipMemcpy(host.re, d_re[cur], n_re * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        }
+        return total_mass<T>(host.re, nb);
     };
     const double cells_i = static_cast<double>(nb[0]) * static_cast<double>(nb[1]) *
                            static_cast<double>(kDim == 3 ? nb[2] : 1);
@@ -347,15 +355,8 @@ int run(const Options& o, const char* argv0)
         }, fname);
     };
 
-    // Initial mass: host compensated sum as the reference (hydro2d/main.cxx:662), or the
-    // device compensated sum when no host copy is needed (--no-output).
-    double initial_mass;
-    if (o.no_output) {
-        initial_mass = device_mass(0);
-    } else {
-        fetch(0);
-        initial_mass = total_mass<T>(host.re, nb);
-    }
+    // Initial mass: the host compensated sum, as the reference (hydro2d/main.cxx:662).
+    const double initial_mass = host_mass(0);
     fprintf(stderr, "\nInit conditions : Diffusion of a 100 times higher pressurized circle at simulation center\n");
     fprintf(stderr, "  Default density      : %.2f inside, %.2f outside\n", static_cast<double>(rho_in),
             static_cast<double>(rho_out));
@@ -441,8 +442,7 @@ int run(const Options& o, const char* argv0)
             T_.compute.push_back(us);
             T_.total.push_back(now_us() - u0);
             if (want_write(it, t)) {
-                fetch(cur);
-                mass_line(initial_mass, total_mass<T>(host.re, nb), it, t);
+                mass_line(initial_mass, host_mass(cur), it, t);
                 write_csv(it, t);
             }
             it_line(it, t, static_cast<double>(Dt), T_.total.back(), us);
@@ -488,8 +488,7 @@ int run(const Options& o, const char* argv0)
                 T_.total.push_back(per_step_us);
                 T_.compute.push_back(comp_us);
                 if (i + 1 == it_now && want_write(i, tt[i])) {
-                    fetch(curb);
-                    mass_line(initial_mass, total_mass<T>(host.re, nb), i, tt[i]);
+                    mass_line(initial_mass, host_mass(curb), i, tt[i]);
                     write_csv(i, tt[i]);
                 }
                 it_line(i, tt[i], td[i], per_step_us, comp_us);
@@ -511,8 +510,7 @@ int run(const Options& o, const char* argv0)
     }
     const double simu_t2 = now_us();
 
-    if (!o.no_output) fetch(cur);
-    const double final_mass = o.no_output ? device_mass(cur) : total_mass<T>(host.re, nb);
+    const double final_mass = host_mass(cur);
     const double final_change = mass_line(initial_mass, final_mass, it, t);
     write_csv(it, t);
     host.writer.join();

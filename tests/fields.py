@@ -43,3 +43,19 @@ def field_errors(got, want):
         l2 = float(np.sqrt(np.mean(d * d)) / scale)
         out[k] = (linf, l2)
     return out
+
+
+def own_scale_velocity_errors(got, want, floor: float = 1e-6):
+    """{component: rel L-inf against the component's own maximum} for the velocity
+    components whose maximum in `want` is at least `floor` times the signal-speed scale;
+    a component below that holds only round-off (it vanishes by symmetry) and is left
+    out (None)."""
+    a, b = primitives(*got), primitives(*want)
+    vscale = signal_speed_scale(b)
+    out = {}
+    for k in ("u", "v", "w"):
+        if k not in b:
+            continue
+        own = float(np.max(np.abs(b[k])))
+        out[k] = float(np.max(np.abs(a[k] - b[k])) / own) if own >= floor * vscale else None
+    return out

@@ -150,3 +150,51 @@ def test_more_streams_than_the_state_pool(gpu):
         torch.cuda.synchronize()
         for h in streams:
             hip.hipStreamDestroy(h.value)
+
+
+def test_stream_states_are_released(gpu):
+    """More streams than the pool of 256 reduction states: each steps once (the streams
+    past the pool get allocated states), then hydro_stream_release gives every state
+    back -- pooled ones to the pool, allocated ones freed -- and a stream created
+    afterwards steps with a recycled state and gets the same Dt as the others."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [C.c_void_p]
+    hip.hipStreamDestroy.argtypes = [C.c_void_p]
+    nb = (20, 30)
+    rhoE, mom = random_state_2d(*nb, seed=21)
+    r, m = to_dev(rhoE), to_dev(mom)
+    out = _outputs(r, m)
+    _step(gpu, 2, nb, r, m, out, None)
+    want = to_host(out[2])[0]
+    streams = []
+    for _ in range(300):
+        h = C.c_void_p()
+        assert hip.hipStreamCreate(C.byref(h)) == 0
+        streams.append(h.value)
+        # A new stream may get the handle of one destroyed earlier in the process whose
+        # state is still bound (the case the release call is for): unbind it first.
+        gpu.stream_release(h.value)
+    bound0, free0 = gpu.stream_states()
+    outs = []
+    for h in streams:
+        outs.append(_outputs(r, m))
+        _step(gpu, 2, nb, r, m, outs[-1], h)
+    torch.cuda.synchronize()
+    bound1, free1 = gpu.stream_states()
+    assert bound1 == bound0 + 300 and free1 == max(0, free0 - 300), (bound0, free0, bound1, free1)
+    assert all(to_host(o[2])[0] == want for o in outs)
+    for h in streams:
+        gpu.stream_release(h)
+        assert hip.hipStreamDestroy(C.c_void_p(h)) == 0
+    assert gpu.stream_states() == (bound0, free0)
+    h = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(h)) == 0
+    o = _outputs(r, m)
+    _step(gpu, 2, nb, r, m, o, h.value)
+    torch.cuda.synchronize()
+    assert to_host(o[2])[0] == want
+    gpu.stream_release(h.value)
+    assert hip.hipStreamDestroy(h) == 0
+    assert gpu.stream_states() == (bound0, free0)

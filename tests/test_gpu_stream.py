@@ -1,4 +1,4 @@
-"""The stream reference kernels (copy, read, fill) touch every byte exactly once, for
+"""The stream reference kernels (copy, non-temporal copy, read, fill) touch every byte exactly once, for
 sizes that end inside a block's chunk, on a chunk boundary, and below one block."""
 import pytest
 import torch
@@ -6,12 +6,13 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("nontemporal", [False, True])
 @pytest.mark.parametrize("n16", [1, 7, 16384, 16384 + 5, 3 * 16384, 5 * 16384 - 1, 1 << 20])
-def test_stream_copy_exact(gpu, n16):
+def test_stream_copy_exact(gpu, n16, nontemporal):
     g = torch.Generator(device="cuda").manual_seed(n16)
     src = torch.randint(-(2**62), 2**62, (2 * n16 + 2,), dtype=torch.int64, device="cuda", generator=g)
     dst = torch.full_like(src, -1)
-    gpu.stream_copy(dst[: 2 * n16], src[: 2 * n16])
+    gpu.stream_copy(dst[: 2 * n16], src[: 2 * n16], nontemporal=nontemporal)
     torch.cuda.synchronize()
     assert torch.equal(dst[: 2 * n16], src[: 2 * n16])
     # Nothing past the end is written.
