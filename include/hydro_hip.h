@@ -167,6 +167,28 @@ int hydro_run_kernel_time(hydro_run* run, double* total_ms, size_t* launches);
 
 void hydro_run_destroy(hydro_run* run);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (tight serving loops, harnesses that forbid hipMalloc; it waits for its stream, so it
+ * is not for use under graph capture): the resident loop of hydro_run_create +
+ * hydro_run_advance + hydro_run_status + hydro_run_trajectory in one synchronous call
+ * that calls no hipMalloc, hipFree or other allocating routine.  Arguments are those of
+ * hydro_run_create and hydro_run_advance; the call runs the loop to its end (at most
+ * max_it steps from buffer a), waits for `stream`, and reports what hydro_run_status
+ * reports (it, t, the current Dt, the buffer holding the current state: 0 = a, 1 = b)
+ * and, when traj_t / traj_dt are not NULL, the it recorded (t, Dt) pairs into those host
+ * arrays.  The results are bit-identical to the allocating entry points.
+ * Its loop state lives in a static device array of the library (1.25 MiB, one per device
+ * used, kept until the process ends), which limits max_it to HYDRO_NOALLOC_MAX_IT: a
+ * larger max_it is rejected with HYDRO_ERR_ARG.  Every call on a device shares that
+ * device's array: calls that overlap on one device (two host threads, or two streams)
+ * race and must not be made; successive calls are independent (each call resets the
+ * state it uses).  Returns 0, HYDRO_ERR_NONFINITE as hydro_run_status, or an error. */
+#define HYDRO_NOALLOC_MAX_IT 65536
+int hydro_run_noalloc(int dim, const size_t* nb, const double* D, double C_min, double dt0, double max_t,
+                      size_t max_it, int precision, const int* bc, void* rhoE_a, void* mom_a, void* rhoE_b,
+                      void* mom_b, size_t* it, double* t, double* dt, int* current, double* traj_t, double* traj_dt,
+                      void* stream);
+
 /* Tuning knobs of the launch geometry (no effect on results), process-wide:
  *   "rows2d" = maximum X rows per wavefront strip of the 2D kernel (4..4096);
  *   "occ2d"  = workgroups per CU of the 2D kernel (3..8, enforced by reserving LDS;

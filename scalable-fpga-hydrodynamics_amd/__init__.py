@@ -82,6 +82,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_run_destroy.restype = None
     L.hydro_run_set_timing.argtypes = [vp, i32]
     L.hydro_run_kernel_time.argtypes = [vp, vp, vp]
+    L.hydro_run_noalloc.argtypes = [i32, vp, vp, d, d, d, sz, i32, vp] + [vp] * 4 + [vp] * 6 + [vp]
     L.hydro_tune_set.argtypes = [C.c_char_p, i32]
     L.hydro_tune_get.argtypes = [C.c_char_p]
     L.hydro_total_mass.argtypes = [vp, sz, sz, sz, i32, vp, vp]

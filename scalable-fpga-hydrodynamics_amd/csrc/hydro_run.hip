@@ -12,6 +12,7 @@
 // adds a ghost-fill launch.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <new>
 #include <stdlib.h>
 #include <string.h>
@@ -65,32 +66,85 @@ void physics_constants(int precision, double C_min, double out[5])
     }
 }
 
+// The loop control at the start of a run (hydro2d/main.cxx:686-689): Dt = dt0 (rounded
+// to the state precision), t = 0, it = 0, and done already when the loop condition fails.
+StepControl initial_control(int dim, int precision, const double* D, const double consts[5], double dt0, double max_t,
+                            size_t max_it)
+{
+    StepControl h;
+    memset(&h, 0, sizeof(h));
+    const bool f32_state = precision == HYDRO_FP32;
+    h.dt = f32_state ? static_cast<double>(static_cast<float>(dt0)) : dt0;
+    h.t = 0.0;
+    h.max_t = max_t;
+    h.it = 0;
+    h.max_it = max_it;
+    h.C_min = consts[0];
+    for (int d = 0; d < dim; ++d) h.D[d] = f32_state ? static_cast<double>(static_cast<float>(D[d])) : D[d];
+    h.done = (0.0 < max_t && 0 < max_it) ? 0u : 1u;  // hydro2d/main.cxx:689
+    return h;
+}
+
+// Checks shared by hydro_run_create and hydro_run_noalloc; `what` names the caller.
+bool run_arguments_ok(const char* what, int dim, const size_t* nb, const double* D, int precision, const int* bc)
+{
+    if ((dim != 2 && dim != 3) || !nb || !D ||
+        (precision != HYDRO_FP64 && precision != HYDRO_FP32 && precision != HYDRO_FP32_FLUX)) {
+        set_error("%s: bad dim/precision/pointer arguments", what);
+        return false;
+    }
+    for (int d = 0; d < dim; ++d) {
+        if (nb[d] < 1) {
+            set_error("%s: nb[%d] must be >= 1", what, d);
+            return false;
+        }
+        if (bc && bc[d] != HYDRO_BC_PERIODIC && bc[d] != HYDRO_BC_REFLECTIVE) {
+            set_error("%s: unknown boundary mode %d on axis %d", what, bc[d], d);
+            return false;
+        }
+    }
+    return true;
+}
+
+// Enqueue one resident step on s: the reflective X ghost fill when X is not periodic
+// (periodic axes are read from the opposite interior plane inside the step kernel, and
+// Y/Z walls are applied inside it), the step kernel and the finalise kernel.
+int enqueue_step(int dim, const size_t* nb, const int* bc, int precision, const double* consts, StepControl* ctl,
+                 double* traj_t, void* traj_dt, void* cr, void* cm, void* nr, void* nm, hipStream_t s,
+                 hipEvent_t ev_a, hipEvent_t ev_b)
+{
+    const bool px = bc[0] == HYDRO_BC_PERIODIC;
+    const int wraps = (px ? kWrapX : 0) | (bc[1] == HYDRO_BC_PERIODIC ? kWrapY : 0) |
+                      (dim == 3 && bc[2] == HYDRO_BC_PERIODIC ? kWrapZ : 0);
+    int rc = 0;
+    if (dim == 2) {
+        if (!px) rc = ghost2d_any(cr, cm, nb[0], nb[1], bc[0], HYDRO_BC_REFLECTIVE, precision, ctl, s);
+        if (!rc)
+            rc = step2d_resident(cr, cm, nr, nm, nb[0], nb[1], bc[1], precision, consts, ctl, &ctl->red, traj_t,
+                                 traj_dt, s, ev_a, ev_b, wraps);
+    } else {
+        if (!px)
+            rc = ghost3d_any(cr, cm, nb[0], nb[1], nb[2], bc[0], HYDRO_BC_REFLECTIVE, HYDRO_BC_REFLECTIVE, precision,
+                             ctl, s);
+        if (!rc)
+            rc = step3d_resident(cr, cm, nr, nm, nb[0], nb[1], nb[2], bc[1], bc[2], precision, consts, ctl, &ctl->red,
+                                 traj_t, traj_dt, s, ev_a, ev_b, wraps);
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" hydro_run* hydro_run_create(int dim, const size_t* nb, const double* D, double C_min, double dt0,
                                        double max_t, size_t max_it, int precision, const int* bc)
 try {
     clear_error();
-    if ((dim != 2 && dim != 3) || !nb || !D ||
-        (precision != HYDRO_FP64 && precision != HYDRO_FP32 && precision != HYDRO_FP32_FLUX)) {
-        set_error("hydro_run_create: bad dim/precision/pointer arguments");
-        return nullptr;
-    }
+    if (!run_arguments_ok("hydro_run_create", dim, nb, D, precision, bc)) return nullptr;
     // The trajectory arrays hold max_it entries of 8 bytes; a count whose size overflows
     // would give a short buffer that the steps then write past.
     if (max_it > SIZE_MAX / sizeof(double)) {
         set_error("hydro_run_create: max_it %zu is too large for its trajectory arrays", max_it);
         return nullptr;
-    }
-    for (int d = 0; d < dim; ++d) {
-        if (nb[d] < 1) {
-            set_error("hydro_run_create: nb[%d] must be >= 1", d);
-            return nullptr;
-        }
-        if (bc && bc[d] != HYDRO_BC_PERIODIC && bc[d] != HYDRO_BC_REFLECTIVE) {
-            set_error("hydro_run_create: unknown boundary mode %d on axis %d", bc[d], d);
-            return nullptr;
-        }
     }
     hydro_run* r = new (std::nothrow) hydro_run();
     if (!r) {
@@ -106,17 +160,8 @@ try {
     r->max_it = max_it;
     physics_constants(precision, C_min, r->consts);
 
-    StepControl h;
-    memset(&h, 0, sizeof(h));
+    const StepControl h = initial_control(dim, precision, D, r->consts, dt0, max_t, max_it);
     const bool f32_state = precision == HYDRO_FP32;
-    h.dt = f32_state ? static_cast<double>(static_cast<float>(dt0)) : dt0;
-    h.t = 0.0;
-    h.max_t = max_t;
-    h.it = 0;
-    h.max_it = max_it;
-    h.C_min = r->consts[0];
-    for (int d = 0; d < dim; ++d) h.D[d] = f32_state ? static_cast<double>(static_cast<float>(D[d])) : D[d];
-    h.done = (0.0 < max_t && 0 < max_it) ? 0u : 1u;  // hydro2d/main.cxx:689
     const size_t traj_bytes_dt = (max_it ? max_it : 1) * (f32_state ? 4 : 8);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->d_ctl), sizeof(StepControl));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&r->d_traj_t), (max_it ? max_it : 1) * sizeof(double));
@@ -160,27 +205,8 @@ try {
         }
         hipEvent_t ev_a = r->timing ? r->ev[2 * r->ev_used] : nullptr;
         hipEvent_t ev_b = r->timing ? r->ev[2 * r->ev_used + 1] : nullptr;
-        // Periodic axes are read from the opposite interior plane inside the step kernel;
-        // only a reflective X boundary needs its mirrored ghost planes filled first (the
-        // Y and Z walls are applied inside the kernel).
-        const bool px = r->bc[0] == HYDRO_BC_PERIODIC;
-        const int wraps = (px ? kWrapX : 0) | (r->bc[1] == HYDRO_BC_PERIODIC ? kWrapY : 0) |
-                          (r->dim == 3 && r->bc[2] == HYDRO_BC_PERIODIC ? kWrapZ : 0);
-        rc = 0;
-        if (r->dim == 2) {
-            if (!px) rc = ghost2d_any(cr, cm, r->nb[0], r->nb[1], r->bc[0], HYDRO_BC_REFLECTIVE, r->precision, r->d_ctl, s);
-            if (!rc)
-                rc = step2d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->bc[1], r->precision, r->consts, r->d_ctl,
-                                     &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b, wraps);
-        } else {
-            if (!px)
-                rc = ghost3d_any(cr, cm, r->nb[0], r->nb[1], r->nb[2], r->bc[0], HYDRO_BC_REFLECTIVE,
-                                 HYDRO_BC_REFLECTIVE, r->precision, r->d_ctl, s);
-            if (!rc)
-                rc = step3d_resident(cr, cm, nr, nm, r->nb[0], r->nb[1], r->nb[2], r->bc[1], r->bc[2], r->precision,
-                                     r->consts, r->d_ctl, &r->d_ctl->red, r->d_traj_t, r->d_traj_dt, s, ev_a, ev_b,
-                                     wraps);
-        }
+        rc = enqueue_step(r->dim, r->nb, r->bc, r->precision, r->consts, r->d_ctl, r->d_traj_t, r->d_traj_dt, cr, cm,
+                          nr, nm, s, ev_a, ev_b);
         if (!rc && ev_b) ++r->ev_used;
         if (rc) return rc;
         ++r->enqueued;
@@ -312,6 +338,160 @@ try {
     delete r;
 } catch (...) {
     hydro::set_exception_error(__func__);
+}
+
+// ---------------------------------------------------------------------------------
+// The resident loop without allocation (hydro_run_noalloc).  Its loop control (Dt, t,
+// iteration, stop flag and the Dt reduction slots) and its (t, Dt) record live in one
+// static device array of the library, `scratch`, instead of memory allocated per run:
+//   [0, 1 KiB)                      StepControl
+//   then kNoAllocMaxIt doubles      t per iteration
+//   then kNoAllocMaxIt doubles      Dt per iteration (FP64 and FP32-flux state)
+//   then kNoAllocMaxIt floats       Dt per iteration as the FP32 state's steps write it
+// 1.25 MiB in all, held by every device the library runs on until the process ends.
+// Host code never passes `scratch` itself (in host code the name is a stand-in, not a
+// device address): each device's address comes from hipGetSymbolAddress, taken once per
+// device with that device current.
+// ---------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocMaxIt = HYDRO_NOALLOC_MAX_IT;
+constexpr size_t kCtlBytes = 1024;
+constexpr size_t kTrajTOff = kCtlBytes;
+constexpr size_t kTrajDtOff = kTrajTOff + kNoAllocMaxIt * sizeof(double);
+constexpr size_t kTrajDt32Off = kTrajDtOff + kNoAllocMaxIt * sizeof(double);
+constexpr size_t kScratchBytes = kTrajDt32Off + kNoAllocMaxIt * sizeof(float);
+static_assert(sizeof(StepControl) <= kCtlBytes, "StepControl outgrew its scratch region");
+static_assert(kTrajTOff % 256 == 0 && kTrajDtOff % 256 == 0 && kTrajDt32Off % 256 == 0, "scratch regions aligned");
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's scratch (taken on the first call on that
+// device), or NULL with the error recorded.
+unsigned char* scratch_address()
+{
+    const int dev = current_device();
+    if (dev < 0) return nullptr;
+    if (dev >= kMaxDevices) {
+        set_error("hydro_run_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hydro_run_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Write the loop control at the start of a call, reduction slots included (one wave,
+// lane q clears slot q): the scratch keeps whatever the previous call left.
+__global__ void __launch_bounds__(kWave) init_control_kernel(StepControl* ctl, StepControl h)
+{
+    ctl->red.slot[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        ctl->dt = h.dt;
+        ctl->t = h.t;
+        ctl->max_t = h.max_t;
+        ctl->it = h.it;
+        ctl->max_it = h.max_it;
+        ctl->C_min = h.C_min;
+        ctl->D[0] = h.D[0];
+        ctl->D[1] = h.D[1];
+        ctl->D[2] = h.D[2];
+        ctl->done = h.done;
+        ctl->nonfinite = h.nonfinite;
+    }
+}
+
+// The FP32 state's Dt record (floats) widened to doubles, in the scratch.
+__global__ void widen_kernel(const float* in, double* out, unsigned long long n)
+{
+    const unsigned long long q = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q < n) out[q] = static_cast<double>(in[q]);
+}
+
+}  // namespace
+
+extern "C" int hydro_run_noalloc(int dim, const size_t* nb, const double* D, double C_min, double dt0, double max_t,
+                                 size_t max_it, int precision, const int* bc, void* rhoE_a, void* mom_a,
+                                 void* rhoE_b, void* mom_b, size_t* it, double* t, double* dt, int* current,
+                                 double* traj_t, double* traj_dt, void* stream)
+{
+    clear_error();
+    if (!run_arguments_ok("hydro_run_noalloc", dim, nb, D, precision, bc)) return HYDRO_ERR_ARG;
+    if (!rhoE_a || !mom_a || !rhoE_b || !mom_b) {
+        set_error("hydro_run_noalloc: null state buffer");
+        return HYDRO_ERR_ARG;
+    }
+    if (max_it > kNoAllocMaxIt) {
+        set_error("hydro_run_noalloc: max_it %zu exceeds the %zu iterations its scratch records", max_it,
+                  kNoAllocMaxIt);
+        return HYDRO_ERR_ARG;
+    }
+    unsigned char* base = scratch_address();
+    if (!base) return HYDRO_ERR_HIP;
+    StepControl* ctl = reinterpret_cast<StepControl*>(base);
+    double* d_traj_t = reinterpret_cast<double*>(base + kTrajTOff);
+    double* d_traj_dt = reinterpret_cast<double*>(base + kTrajDtOff);
+    float* d_traj_dt32 = reinterpret_cast<float*>(base + kTrajDt32Off);
+    const bool f32_state = precision == HYDRO_FP32;
+
+    int bcs[3] = { HYDRO_BC_PERIODIC, HYDRO_BC_REFLECTIVE, HYDRO_BC_REFLECTIVE };
+    for (int d = 0; d < dim; ++d)
+        if (bc) bcs[d] = bc[d];
+    double consts[5];
+    physics_constants(precision, C_min, consts);
+    const StepControl h = initial_control(dim, precision, D, consts, dt0, max_t, max_it);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(init_control_kernel, dim3(1), dim3(kWave), 0, s, ctl, h);
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    // max_it launches; those after the loop has ended are no-ops (hydro_run_advance).
+    for (size_t n = 0; n < max_it; ++n) {
+        const bool in_a = (n % 2) == 0;
+        const int rc = enqueue_step(dim, nb, bcs, precision, consts, ctl, d_traj_t,
+                                    f32_state ? static_cast<void*>(d_traj_dt32) : static_cast<void*>(d_traj_dt),
+                                    in_a ? rhoE_a : rhoE_b, in_a ? mom_a : mom_b, in_a ? rhoE_b : rhoE_a,
+                                    in_a ? mom_b : mom_a, s, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    StepControl out;
+    HYDRO_HIP_OR(hipMemcpyAsync(&out, ctl, sizeof(out), hipMemcpyDeviceToHost, s), HYDRO_ERR_HIP);
+    HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
+    const size_t done_it = static_cast<size_t>(out.it);
+    if (done_it > 0 && traj_t)
+        HYDRO_HIP_OR(hipMemcpyAsync(traj_t, d_traj_t, done_it * sizeof(double), hipMemcpyDeviceToHost, s),
+                     HYDRO_ERR_HIP);
+    if (done_it > 0 && traj_dt) {
+        if (f32_state) {
+            const unsigned blocks = static_cast<unsigned>((done_it + 255) / 256);
+            hipLaunchKernelGGL(widen_kernel, dim3(blocks), dim3(256), 0, s, d_traj_dt32, d_traj_dt,
+                               static_cast<unsigned long long>(done_it));
+            HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+        }
+        HYDRO_HIP_OR(hipMemcpyAsync(traj_dt, d_traj_dt, done_it * sizeof(double), hipMemcpyDeviceToHost, s),
+                     HYDRO_ERR_HIP);
+    }
+    HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
+    if (it) *it = done_it;
+    if (t) *t = out.t;
+    if (dt) *dt = out.dt;
+    if (current) *current = static_cast<int>(out.it % 2);
+    if (out.nonfinite) {
+        set_error("hydro_run_noalloc: Dt became non-finite at iteration %llu (NaN or zero signal speed)",
+                  static_cast<unsigned long long>(out.it ? out.it - 1 : 0));
+        return HYDRO_ERR_NONFINITE;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------------------------
