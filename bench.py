@@ -403,6 +403,17 @@ def _latency_by_dim(reqs, lat) -> dict:
     return out
 
 
+def _large_class(reqs, lat, seq_lat, dur, large_s) -> dict:
+    """Latencies of the requests whose measured plain solve time is at least large_s
+    (about the server's large size class, which it picks from an estimate), served and
+    one at a time."""
+    sel = [i for i in range(len(reqs)) if large_s > 0 and dur[i] >= large_s]
+    if not sel:
+        return {"requests": 0}
+    return {"requests": len(sel), "served_max_s": round(max(lat[i] for i in sel), 4),
+            "sequential_max_s": round(max(seq_lat[i] for i in sel), 4)}
+
+
 def bench_served(args) -> dict:
     """Poisson stream of mixed solves (70% 2D sides 128..4096, 30% 3D sides 64..256,
     100..2000 steps, FP32-flux / FP64 50/50, seeded blasts) through the continuous-batching
@@ -442,11 +453,19 @@ def bench_served(args) -> dict:
     log(f"[bench] served at {rate:.2f} requests/s (80% of the sequential server) ...")
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
     srv.set_limits(args.serve_cap_cells, args.steps_per_tick)
+    srv.set_policy(args.serve_large_ms / 1e3)
     live = sv.run_trace(hydro, srv, reqs, realtime=True, progress=prog)
     srv.close()
+    real_seq = None
+    if not args.no_real_sequential:
+        log("[bench] the one-at-a-time server run for real at the same arrivals ...")
+        torch.cuda.empty_cache()
+        real_seq = sv.run_sequential(hydro, torch, reqs, progress=prog)
+        torch.cuda.empty_cache()
     log("[bench] served at saturation (all requests at t = 0) ...")
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
     srv.set_limits(args.serve_cap_cells, args.steps_per_tick)
+    srv.set_policy(args.serve_large_ms / 1e3)
     sat = sv.run_trace(hydro, srv, reqs, realtime=False, progress=prog)
     srv.close()
     seq_gcups = work / sum(dur) / 1e9
@@ -466,6 +485,7 @@ def bench_served(args) -> dict:
         "config": {"workload": "served_mixed", "requests": n, "mix": "70% 2D 128..4096, 30% 3D 64..256",
                    "steps": "100..2000", "precision": "50% fp32-flux / 50% fp64", "hbm_arena_gb": args.serve_hbm_gb,
                    "steps_per_tick": args.steps_per_tick, "engine_cap_cells": args.serve_cap_cells,
+                   "large_class_ms": args.serve_large_ms,
                    "arrival_rate_per_s": round(rate, 3),
                    "load": "0.8 of the one-at-a-time plain server"},
         "served": {"p50_s": round(live["p50"], 4), "p99_s": round(live["p99"], 4), "gcups": round(live["gcups"], 3),
@@ -474,12 +494,17 @@ def bench_served(args) -> dict:
                              "p99_s": round(sv.percentile(seq_lat, 99), 4),
                              "gcups": round(work / (seq_finish - reqs[0]["arrival"]) / 1e9, 3),
                              "mean_solve_s": round(mean_d, 4)},
+        "sequential_plain_real": ({"p50_s": round(real_seq["p50"], 4), "p99_s": round(real_seq["p99"], 4),
+                                   "gcups": round(real_seq["gcups"], 3),
+                                   "note": "the same one-at-a-time server run in real time, IC and buffers included"}
+                                  if real_seq else None),
         "sequential_sync_launcher": {"p50_s": round(sv.percentile(sync_lat, 50), 4),
                                      "p99_s": round(sv.percentile(sync_lat, 99), 4),
                                      "gcups": round(work / (sync_finish - reqs[0]["arrival"]) / 1e9, 3),
                                      "mean_solve_s": round(sum(dur_sync) / n, 4)},
         "by_dimension": {"served": _latency_by_dim(reqs, live["latency"]),
                          "sequential_plain": _latency_by_dim(reqs, seq_lat)},
+        "large_class": _large_class(reqs, live["latency"], seq_lat, dur, args.serve_large_ms / 1e3),
         "saturation": {"served_gcups": round(sat["gcups"], 3), "sequential_plain_gcups": round(seq_gcups, 3),
                        "speedup": round(sat["gcups"] / seq_gcups, 3),
                        "sequential_sync_launcher_gcups": round(work / sum(dur_sync) / 1e9, 3),
@@ -489,8 +514,9 @@ def bench_served(args) -> dict:
         "cpu_baseline": None,
     }
     log(f"[bench] served p50 {live['p50']:.3f} s p99 {live['p99']:.3f} s | sequential p50 "
-        f"{line['sequential_plain']['p50_s']} s p99 {line['sequential_plain']['p99_s']} s | saturation "
-        f"{sat['gcups']:.1f} vs {seq_gcups:.1f} GCUPS")
+        f"{line['sequential_plain']['p50_s']} s p99 {line['sequential_plain']['p99_s']} s"
+        + (f" (real run: p50 {real_seq['p50']:.3f} s p99 {real_seq['p99']:.3f} s)" if real_seq else "")
+        + f" | saturation {sat['gcups']:.1f} vs {seq_gcups:.1f} GCUPS")
     return line
 
 
@@ -543,6 +569,11 @@ def main() -> int:
     ap.add_argument("--requests", type=int, default=120, help="served workload: number of requests")
     ap.add_argument("--serve-hbm-gb", type=float, default=48.0, help="served workload: state arena (GiB)")
     ap.add_argument("--steps-per-tick", type=int, default=20)
+    ap.add_argument("--serve-large-ms", type=float, default=20.0,
+                    help="served workload: requests whose estimated solve time alone is at least this many ms run "
+                         "one at a time in arrival order beside the smaller ones (0: one class)")
+    ap.add_argument("--no-real-sequential", action="store_true",
+                    help="served workload: skip the real one-at-a-time server run")
     ap.add_argument("--serve-cap-cells", type=float, default=6e6,
                     help="served workload: per-engine cap on the interior cells of running requests (0 = none)")
     ap.add_argument("--csv-grid", type=int, nargs=2, default=[6000, 5000], help="csv workload: grid of the driver run")
@@ -787,8 +818,10 @@ def main() -> int:
             "served_mixed": {"requests": sv["config"]["requests"], "arrival_rate_per_s": sv["config"]["arrival_rate_per_s"],
                              "load": sv["config"]["load"], "served": sv["served"],
                              "sequential_plain": sv["sequential_plain"],
+                             "sequential_plain_real": sv["sequential_plain_real"],
                              "sequential_sync_launcher": sv["sequential_sync_launcher"],
-                             "by_dimension": sv["by_dimension"], "saturation": sv["saturation"]},
+                             "by_dimension": sv["by_dimension"], "large_class": sv["large_class"],
+                             "large_class_ms": sv["config"]["large_class_ms"], "saturation": sv["saturation"]},
         }
     print(json.dumps(line), flush=True)
     return 0

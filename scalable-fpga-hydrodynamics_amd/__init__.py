@@ -124,6 +124,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_server_release.argtypes = [vp, i32]
     L.hydro_server_stats.argtypes = [vp, vp, vp]
     L.hydro_server_set_limits.argtypes = [vp, d, sz]
+    L.hydro_server_set_policy.argtypes = [vp, d]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
 
@@ -639,6 +640,12 @@ class Server:
     def set_limits(self, max_active_cells: float = 0.0, steps_per_tick: int = 50) -> None:
         _check(lib().hydro_server_set_limits(self._h, float(max_active_cells), int(steps_per_tick)),
                "hydro_server_set_limits")
+
+    def set_policy(self, large_seconds: float = 0.0) -> None:
+        """Requests whose estimated solve time alone is at least large_seconds run one at
+        a time in arrival order as plain resident solves, beside the smaller ones in the
+        batched engines (0: one class)."""
+        _check(lib().hydro_server_set_policy(self._h, float(large_seconds)), "hydro_server_set_policy")
 
     def stats(self):
         t, cu = C.c_size_t(), C.c_double()

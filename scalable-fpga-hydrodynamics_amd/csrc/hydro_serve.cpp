@@ -16,6 +16,16 @@
 // engine with work is still busy.  A small engine therefore never waits for a large
 // one.  Latency = completion observed after the request's last tick - arrival, on the
 // server's steady clock.
+//
+// Size classes (hydro_server_set_policy): a request whose estimated solve time alone
+// (its steps times a per-step cost of the plain resident loop on MI355X, launch floor
+// plus cells) reaches `large_seconds` is large.  Large requests run one at a time, in
+// arrival order, each as a plain device-resident solve (hydro_run, every step enqueued
+// at admission) on a stream of their own; smaller requests go to the batched engines at
+// once, beside it.  Sharing the device among everything admitted (the default, one
+// class) stretches the largest solves by the load, and a plain FIFO makes small solves
+// wait behind large ones; the size classes keep the large solves close to their FIFO
+// latency while the small ones overtake them.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -101,7 +111,19 @@ struct Req {
     size_t it = 0;
     double t_sim = 0;
     double interior = 0;  // interior cells
+    bool large = false;   // in the large size class (runs alone among large requests)
 };
+
+// Estimated time of a request's solve alone on the plain resident loop: per step a
+// launch floor plus a cost per interior cell, fitted to plain solves of the served mix on
+// MI355X (2D: 9 us + 13 ps per cell, i.e. about 4.9 TB/s at 64 B per cell; 3D: 11 us +
+// 24 ps per cell, about 3.4 TB/s at 80 B).  Only ranks requests into the size classes.
+double solo_seconds(int dim, double interior, size_t steps)
+{
+    const double floor_s = dim == 3 ? 11e-6 : 9e-6;
+    const double per_cell = dim == 3 ? 24e-12 : 13e-12;
+    return static_cast<double>(steps) * (floor_s + per_cell * interior);
+}
 
 size_t cells_of(const Req& r)
 {
@@ -110,6 +132,8 @@ size_t cells_of(const Req& r)
     return c;
 }
 
+// Batched engines, one per (dimension, precision); large requests use none.
+constexpr int kEngines = 4;
 int engine_of(const Req& r) { return (r.dim == 3 ? 2 : 0) + (r.precision == HYDRO_FP32_FLUX ? 1 : 0); }
 
 }  // namespace
@@ -120,18 +144,24 @@ struct hydro_server {
     Arena arena;
     size_t capacity = 0;  // running requests per engine
     size_t steps_per_tick = 50;
-    hydro_batch* eng[4] = { nullptr, nullptr, nullptr, nullptr };
-    hipEvent_t ev[4] = { nullptr, nullptr, nullptr, nullptr };  // end of the engine's last tick
-    bool busy[4] = { false, false, false, false };
-    std::vector<int> slot_req[4];  // slot -> request id (-1 free)
-    size_t running[4] = { 0, 0, 0, 0 };
+    hydro_batch* eng[kEngines] = {};
+    hipEvent_t ev[kEngines] = {};  // end of the engine's last tick
+    bool busy[kEngines] = {};
+    std::vector<int> slot_req[kEngines];  // slot -> request id (-1 free)
+    size_t running[kEngines] = {};
     std::vector<Req> reqs;
     std::deque<int> queue;
     std::chrono::steady_clock::time_point t0;
     size_t ticks = 0;
     double cell_updates = 0;  // of retired requests
     double max_active_cells = 0;  // admission cap on interior cells in flight per engine (0 = none)
-    double active_cells[4] = { 0, 0, 0, 0 };
+    double active_cells[kEngines] = {};
+    // Size classes: threshold (0: one class) and the one large request running, if any.
+    double large_seconds = 0;
+    int large_id = -1;
+    hydro_run* large_run = nullptr;
+    hipStream_t large_stream = nullptr;
+    hipEvent_t large_ev = nullptr;  // end of the large solve's steps
 
     double now() const
     {
@@ -200,11 +230,77 @@ int ensure_engine(hydro_server* s, int e)
 // full waits for it without holding back requests of other engines; admission stops at
 // the first request whose memory (or the active-cell cap) does not fit, so that later
 // requests never take the memory an earlier one is waiting for.
+// The four arrays of a request in its arena block: rhoE and momentum of buffer a, then
+// of buffer b.
+void buffers_of(hydro_server* s, const Req& r, size_t off, void* p[4])
+{
+    char* b = s->base + off;
+    p[0] = b;
+    p[1] = b + r.buf_bytes[0];
+    p[2] = b + r.buf_bytes[0] + r.buf_bytes[1];
+    p[3] = b + 2 * r.buf_bytes[0] + r.buf_bytes[1];
+}
+
+// Start a large request as a plain resident solve on the large stream: the blast IC of
+// hydro_batch_add_blast (ic_mode 1, D = 1 / (nb + 1), C_min = C * min(D)), then every
+// step enqueued at once, then the completion event.
+int start_large(hydro_server* s, int id, size_t off)
+{
+    Req& r = s->reqs[id];
+    void* p[4];
+    buffers_of(s, r, off, p);
+    double D[3] = { 0, 0, 0 }, ms = 0;
+    for (int d = 0; d < r.dim; ++d) {
+        D[d] = 1.0 / static_cast<double>(r.nb[d] + 1);
+        ms = (d == 0 || D[d] < ms) ? D[d] : ms;
+    }
+    const double C_min = r.C * ms;
+    double dt0 = 0;
+    int rc = hydro_ic(static_cast<double*>(p[0]), static_cast<double*>(p[1]), r.dim, r.nb, nullptr, r.nb, r.rho_in,
+                      r.rho_out, r.p_in, r.p_out, 1, r.rf, C_min, 1.4, &dt0, s->large_stream);
+    if (rc) return rc;
+    hydro_run* run = hydro_run_create(r.dim, r.nb, D, C_min, dt0, r.max_t, r.max_it, r.precision, nullptr);
+    if (!run) return HYDRO_ERR_HIP;
+    rc = hydro_run_advance(run, p[0], p[1], p[2], p[3], r.max_it, s->large_stream);
+    if (rc == 0 && hipEventRecord(s->large_ev, s->large_stream) != hipSuccess) {
+        set_error("hydro server: cannot record the large solve's event");
+        rc = HYDRO_ERR_HIP;
+    }
+    if (rc) {
+        (void)hipStreamSynchronize(s->large_stream);
+        hydro_run_destroy(run);
+        return rc;
+    }
+    s->large_run = run;
+    s->large_id = id;
+    return 0;
+}
+
 int admit(hydro_server* s)
 {
     for (auto q = s->queue.begin(); q != s->queue.end();) {
         const int id = *q;
         Req& r = s->reqs[id];
+        if (r.large) {
+            // One large request at a time, in arrival order (every later large request is
+            // held back too); small ones pass.
+            if (s->large_id >= 0) {
+                ++q;
+                continue;
+            }
+            size_t off = 0;
+            if (!s->arena.alloc(r.bytes, off)) break;
+            const int rc = start_large(s, id, off);
+            if (rc) {
+                s->arena.release(off, r.bytes);
+                return rc;
+            }
+            r.off = off;
+            r.state = kRunning;
+            r.t_admit = s->now();
+            q = s->queue.erase(q);
+            continue;
+        }
         const int e = engine_of(r);
         int rc = ensure_engine(s, e);
         if (rc) return rc;
@@ -219,13 +315,10 @@ int admit(hydro_server* s)
         }
         size_t off = 0;
         if (!s->arena.alloc(r.bytes, off)) break;
-        char* p = s->base + off;
-        void* ra = p;
-        void* ma = p + r.buf_bytes[0];
-        void* rb = p + r.buf_bytes[0] + r.buf_bytes[1];
-        void* mb = p + 2 * r.buf_bytes[0] + r.buf_bytes[1];
+        void* p[4];
+        buffers_of(s, r, off, p);
         const int slot = hydro_batch_add_blast(s->eng[e], r.nb, r.C, r.rf, r.rho_in, r.rho_out, r.p_in, r.p_out,
-                                               r.max_t, r.max_it, ra, ma, rb, mb);
+                                               r.max_t, r.max_it, p[0], p[1], p[2], p[3]);
         if (slot < 0) {
             s->arena.release(off, r.bytes);
             return slot;
@@ -278,6 +371,23 @@ int retire(hydro_server* s, int e, size_t* n_done)
     return 0;
 }
 
+// Retire the large request once its event has completed (the caller checked).
+int retire_large(hydro_server* s, size_t* n_done)
+{
+    Req& r = s->reqs[s->large_id];
+    int done = 0;
+    r.rc = hydro_run_status(s->large_run, &r.it, &r.t_sim, nullptr, &done, &r.current, s->large_stream);
+    r.t_done = s->now();
+    r.state = r.rc == 0 ? kDone : kFailed;
+    s->cell_updates += static_cast<double>(r.it) * r.interior;
+    hydro_run_destroy(s->large_run);
+    s->large_run = nullptr;
+    s->large_id = -1;
+    if (!r.keep || r.state == kFailed) release_memory(s, r);
+    if (n_done) ++*n_done;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" hydro_server* hydro_server_create(size_t hbm_bytes, size_t capacity, size_t steps_per_tick)
@@ -297,9 +407,12 @@ try {
     s->t0 = std::chrono::steady_clock::now();
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->base), hbm_bytes);
+    // The large class's stream (default priority) and completion event.
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->large_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->large_ev, hipEventDisableTiming);
     if (e != hipSuccess) {
         set_error("hydro_server_create: %s (%zu bytes)", hipGetErrorString(e), hbm_bytes);
-        delete s;
+        hydro_server_destroy(s);
         return nullptr;
     }
     try {
@@ -317,10 +430,14 @@ try {
 extern "C" void hydro_server_destroy(hydro_server* s)
 try {
     if (!s) return;
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < kEngines; ++e) {
         if (s->eng[e]) hydro_batch_destroy(s->eng[e]);
         if (s->ev[e]) (void)hipEventDestroy(s->ev[e]);
     }
+    if (s->large_stream) (void)hipStreamSynchronize(s->large_stream);
+    if (s->large_run) hydro_run_destroy(s->large_run);
+    if (s->large_ev) (void)hipEventDestroy(s->large_ev);
+    if (s->large_stream) (void)hipStreamDestroy(s->large_stream);
     if (s->base) (void)hipFree(s->base);
     delete s;
 } catch (...) {
@@ -374,6 +491,7 @@ try {
     }
     r.interior = 1;
     for (int d = 0; d < dim; ++d) r.interior *= static_cast<double>(r.nb[d]);
+    r.large = s->large_seconds > 0 && solo_seconds(dim, r.interior, max_it) >= s->large_seconds;
     r.buf_bytes[0] = (b0 + kAlign - 1) / kAlign * kAlign;
     r.buf_bytes[1] = (b1 + kAlign - 1) / kAlign * kAlign;
     r.bytes = 2 * (r.buf_bytes[0] + r.buf_bytes[1]);
@@ -406,7 +524,7 @@ try {
     // admit) is idle.
     for (;;) {
         bool any_busy = false, any_ready = false;
-        for (int e = 0; e < 4; ++e) {
+        for (int e = 0; e < kEngines; ++e) {
             if (s->busy[e]) {
                 const hipError_t q = hipEventQuery(s->ev[e]);
                 if (q == hipSuccess) s->busy[e] = false;
@@ -418,18 +536,30 @@ try {
             any_busy = any_busy || s->busy[e];
             any_ready = any_ready || (!s->busy[e] && s->running[e] > 0);
         }
+        bool large_done = false;
+        if (s->large_id >= 0) {
+            const hipError_t q = hipEventQuery(s->large_ev);
+            if (q == hipSuccess) large_done = true;
+            else if (q != hipErrorNotReady) {
+                set_error("hydro_server_tick: %s", hipGetErrorString(q));
+                return HYDRO_ERR_HIP;
+            }
+            any_busy = any_busy || !large_done;
+            any_ready = any_ready || large_done;
+        }
         for (const int id : s->queue) {
-            const int e = engine_of(s->reqs[id]);
-            any_ready = any_ready || !s->busy[e];
+            const Req& r = s->reqs[id];
+            any_ready = any_ready || (r.large ? s->large_id < 0 : !s->busy[engine_of(r)]);
         }
         if (any_ready || !any_busy) break;
         std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
     int rc = 0;
-    for (int e = 0; e < 4 && rc == 0; ++e)
+    if (s->large_id >= 0 && hipEventQuery(s->large_ev) == hipSuccess) rc = retire_large(s, n_done);
+    for (int e = 0; e < kEngines && rc == 0; ++e)
         if (!s->busy[e]) rc = retire(s, e, n_done);
     if (rc == 0) rc = admit(s);
-    for (int e = 0; e < 4 && rc == 0; ++e)
+    for (int e = 0; e < kEngines && rc == 0; ++e)
         if (s->eng[e] && !s->busy[e] && s->running[e] > 0) {
             rc = hydro_batch_advance(s->eng[e], s->steps_per_tick);
             if (rc == 0) {
@@ -452,7 +582,10 @@ extern "C" int hydro_server_pending(hydro_server* s, size_t* queued, size_t* run
 try {
     if (!check(s, "hydro_server_pending")) return HYDRO_ERR_ARG;
     if (queued) *queued = s->queue.size();
-    if (running) *running = s->running[0] + s->running[1] + s->running[2] + s->running[3];
+    if (running) {
+        *running = s->large_id >= 0 ? 1 : 0;
+        for (int e = 0; e < kEngines; ++e) *running += s->running[e];
+    }
     if (bytes_in_use) *bytes_in_use = s->arena.used();
     return 0;
 } catch (...) {
@@ -525,6 +658,20 @@ try {
     }
     s->max_active_cells = max_active_cells;
     s->steps_per_tick = steps_per_tick;
+    return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
+}
+
+extern "C" int hydro_server_set_policy(hydro_server* s, double large_seconds)
+try {
+    if (!check(s, "hydro_server_set_policy")) return HYDRO_ERR_ARG;
+    if (!(large_seconds >= 0)) {
+        set_error("hydro_server_set_policy: large_seconds >= 0");
+        return HYDRO_ERR_ARG;
+    }
+    s->large_seconds = large_seconds;  // classifies requests submitted from now on
     return 0;
 } catch (...) {
     hydro::set_exception_error(__func__);

@@ -40,9 +40,20 @@ def _alone(gpu, req):
     return info, out
 
 
-def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle):
+def _solo_estimate(r):
+    """The server's estimate of a request's solve time alone (size classes): per step
+    9 us + 13 ps per interior cell in 2D, 11 us + 24 ps in 3D."""
+    cells = float(np.prod(r["nb"]))
+    return r["steps"] * ((9e-6 + 13e-12 * cells) if len(r["nb"]) == 2 else (11e-6 + 24e-12 * cells))
+
+
+@pytest.mark.parametrize("large_s", [0.0, 9e-4])
+def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle, large_s):
+    """With one class, and with size classes (large requests run one at a time, in
+    arrival order, as plain resident solves beside the batched small ones)."""
     reqs = _mix()
     s = gpu.Server(1 << 30, capacity=16, steps_per_tick=13)
+    s.set_policy(large_s)
     ids = []
     # Staggered arrivals: a few requests, a tick, more requests (they join running work).
     for k, r in enumerate(reqs):
@@ -53,8 +64,14 @@ def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle):
     while any(s.request(i)["state"] < s.DONE for i in ids):
         s.tick()
     kinds = {(len(r["nb"]), r["fp32flux"]) for r in reqs}
-    print(f"[serve] {len(reqs)} requests, kinds {sorted(kinds)}", flush=True)
+    large = [rid for r, rid in zip(reqs, ids) if large_s > 0 and _solo_estimate(r) >= large_s]
+    print(f"[serve] {len(reqs)} requests, kinds {sorted(kinds)}, {len(large)} large", flush=True)
     assert len(kinds) >= 3
+    if large_s > 0:
+        assert 2 <= len(large) < len(reqs)
+        spans = [(s.request(i)["admit"], s.request(i)["done"]) for i in large]
+        for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
+            assert d0 <= a1, (spans,)  # one large request at a time, in arrival order
     for r, rid in zip(reqs, ids):
         info = s.request(rid)
         assert info["rc"] == 0 and info["state"] == s.DONE and info["it"] == r["steps"], info
