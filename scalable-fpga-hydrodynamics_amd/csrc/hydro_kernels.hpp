@@ -40,6 +40,23 @@ int ic_blast_async_f64(double* d_rhoE, double* d_mom, int dim, const size_t* nb,
                        double p_in, double p_out, int ic_mode, double radius_frac, double C_min, double gamma,
                        unsigned long long* d_flags, double* dt_in, double* dt_out, hipStream_t s);
 
+}  // namespace hydro
+
+struct hydro_run;
+
+namespace hydro {
+
+// Restart a run created by hydro_run_create for a new blast solve, with no allocation
+// and no wait: the blast IC of hydro_batch_add_blast (ic_mode 1, D = 1 / (nb + 1),
+// C_min = C * min(D)) written into (rhoE, mom) on s, then the loop control (t = 0,
+// it = 0, Dt from the IC) set on the device after it, X periodic and Y/Z reflective.
+// FP64 or FP32-flux precision; max_it at most run_capacity(r).  The steps then come
+// from hydro_run_advance with (rhoE, mom) as buffer a.
+int run_restart_blast(hydro_run* r, int dim, const size_t* nb, int precision, double C, double radius_frac,
+                      double rho_in, double rho_out, double p_in, double p_out, double max_t, size_t max_it,
+                      void* rhoE, void* mom, hipStream_t s);
+size_t run_capacity(const hydro_run* r);
+
 // Dynamic LDS that leaves room for exactly `occ` workgroups of Kernel per CU (160 KiB of
 // LDS): the middle of the interval (L/(occ+1), L/occ], less the kernel's static LDS, so
 // allocation granularity cannot move the count.  Fewer waves in flight per CU keep fewer

@@ -34,7 +34,9 @@ struct hydro_run {
     StepControl* d_ctl = nullptr;
     double* d_traj_t = nullptr;
     void* d_traj_dt = nullptr;
+    unsigned long long* d_flags = nullptr;  // IC state flags of run_restart_blast
     size_t max_it = 0;
+    size_t cap_it = 0;  // iterations the trajectory record holds
     size_t enqueued = 0;
     // Optional per-launch timing of the step kernel (HIP events on the run's stream).
     bool timing = false;
@@ -158,6 +160,7 @@ try {
         if (bc) r->bc[d] = bc[d];
     }
     r->max_it = max_it;
+    r->cap_it = max_it;
     physics_constants(precision, C_min, r->consts);
 
     const StepControl h = initial_control(dim, precision, D, r->consts, dt0, max_t, max_it);
@@ -166,6 +169,7 @@ try {
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&r->d_ctl), sizeof(StepControl));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&r->d_traj_t), (max_it ? max_it : 1) * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&r->d_traj_dt, traj_bytes_dt);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&r->d_flags), sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemcpy(r->d_ctl, &h, sizeof(h), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_error("hydro_run_create: %s", hipGetErrorString(e));
@@ -335,10 +339,92 @@ try {
     if (r->d_ctl) (void)hipFree(r->d_ctl);
     if (r->d_traj_t) (void)hipFree(r->d_traj_t);
     if (r->d_traj_dt) (void)hipFree(r->d_traj_dt);
+    if (r->d_flags) (void)hipFree(r->d_flags);
     delete r;
 } catch (...) {
     hydro::set_exception_error(__func__);
 }
+
+// ---------------------------------------------------------------------------------
+// Restarting a run for a new blast without allocating or waiting (the served form's
+// large requests, which would otherwise create and destroy a run each: hipFree waits
+// for the whole device).
+// ---------------------------------------------------------------------------------
+namespace {
+
+__global__ void clear_flags_kernel(unsigned long long* flags) { *flags = 0ull; }
+
+// The loop control of a restarted run: h with Dt = min over the IC's states present
+// (flags bit 0: inside, bit 1: outside) of their C_min / speed, as the reference's
+// std::min over cells starting from 1e20 (hydro2d/main.cxx:318-337); reduction slots
+// cleared (one wave, lane q clears slot q).
+__global__ void __launch_bounds__(kWave) start_control_kernel(StepControl* ctl, const unsigned long long* flags,
+                                                              double dt_in, double dt_out, StepControl h)
+{
+    ctl->red.slot[threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) {
+        double m = 1e20;
+        if (*flags & 1ull) m = (dt_in < m) ? dt_in : m;
+        if (*flags & 2ull) m = (dt_out < m) ? dt_out : m;
+        ctl->dt = m;
+        ctl->t = h.t;
+        ctl->max_t = h.max_t;
+        ctl->it = h.it;
+        ctl->max_it = h.max_it;
+        ctl->C_min = h.C_min;
+        ctl->D[0] = h.D[0];
+        ctl->D[1] = h.D[1];
+        ctl->D[2] = h.D[2];
+        ctl->done = h.done;
+        ctl->nonfinite = 0u;
+    }
+}
+
+}  // namespace
+
+namespace hydro {
+
+size_t run_capacity(const hydro_run* r) { return r ? r->cap_it : 0; }
+
+int run_restart_blast(hydro_run* r, int dim, const size_t* nb, int precision, double C, double radius_frac,
+                      double rho_in, double rho_out, double p_in, double p_out, double max_t, size_t max_it,
+                      void* rhoE, void* mom, hipStream_t s)
+{
+    if (!r || !rhoE || !mom || (precision != HYDRO_FP64 && precision != HYDRO_FP32_FLUX) || max_it > r->cap_it) {
+        set_error("run_restart_blast: bad run, buffers, precision or max_it %zu (record of %zu)", max_it,
+                  r ? r->cap_it : 0);
+        return HYDRO_ERR_ARG;
+    }
+    // Spacings and C_min as hydro_batch_add_blast (hydro2d/main.cxx:482-484, 595-607).
+    double D[3] = { 0, 0, 0 }, ms = 0;
+    for (int d = 0; d < dim; ++d) {
+        D[d] = 1.0 / static_cast<double>(nb[d] + 1);
+        ms = (d == 0 || D[d] < ms) ? D[d] : ms;
+    }
+    const double C_min = C * ms;
+    r->dim = dim;
+    r->precision = precision;
+    for (int d = 0; d < 3; ++d) {
+        r->nb[d] = d < dim ? nb[d] : 0;
+        r->bc[d] = d == 0 ? HYDRO_BC_PERIODIC : HYDRO_BC_REFLECTIVE;
+    }
+    r->max_it = max_it;
+    r->enqueued = 0;
+    physics_constants(precision, C_min, r->consts);
+    const StepControl h = initial_control(dim, precision, D, r->consts, 0.0, max_t, max_it);
+    hipLaunchKernelGGL(clear_flags_kernel, dim3(1), dim3(1), 0, s, r->d_flags);
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    double dt_in = 0, dt_out = 0;
+    const int rc = ic_blast_async_f64(static_cast<double*>(rhoE), static_cast<double*>(mom), dim, nb, rho_in,
+                                      rho_out, p_in, p_out, 1, radius_frac, C_min, 1.4, r->d_flags, &dt_in, &dt_out,
+                                      s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(start_control_kernel, dim3(1), dim3(kWave), 0, s, r->d_ctl, r->d_flags, dt_in, dt_out, h);
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    return 0;
+}
+
+}  // namespace hydro
 
 // ---------------------------------------------------------------------------------
 // The resident loop without allocation (hydro_run_noalloc).  Its loop control (Dt, t,

@@ -37,6 +37,7 @@
 #include <vector>
 
 #include "../../include/hydro_hip.h"
+#include "hydro_kernels.hpp"
 #include "hydro_runtime.hpp"
 
 using namespace hydro;
@@ -241,37 +242,42 @@ void buffers_of(hydro_server* s, const Req& r, size_t off, void* p[4])
     p[3] = b + 2 * r.buf_bytes[0] + r.buf_bytes[1];
 }
 
-// Start a large request as a plain resident solve on the large stream: the blast IC of
-// hydro_batch_add_blast (ic_mode 1, D = 1 / (nb + 1), C_min = C * min(D)), then every
-// step enqueued at once, then the completion event.
+// Start a large request as a plain resident solve on the large stream: the server's one
+// resident run is restarted for it (the blast IC of hydro_batch_add_blast, ic_mode 1,
+// D = 1 / (nb + 1), C_min = C * min(D), written on the device), every step is enqueued
+// at once, then the completion event.  Nothing is allocated and nothing waits, except
+// when a request needs a longer (t, Dt) record than the run has (created with at least
+// kLargeRecord iterations), which reallocates it once.
+constexpr size_t kLargeRecord = 4096;
+
 int start_large(hydro_server* s, int id, size_t off)
 {
     Req& r = s->reqs[id];
     void* p[4];
     buffers_of(s, r, off, p);
-    double D[3] = { 0, 0, 0 }, ms = 0;
-    for (int d = 0; d < r.dim; ++d) {
-        D[d] = 1.0 / static_cast<double>(r.nb[d] + 1);
-        ms = (d == 0 || D[d] < ms) ? D[d] : ms;
+    if (!s->large_run || hydro::run_capacity(s->large_run) < r.max_it) {
+        if (s->large_run) {
+            (void)hipStreamSynchronize(s->large_stream);
+            hydro_run_destroy(s->large_run);
+            s->large_run = nullptr;
+        }
+        const size_t nb1[3] = { 1, 1, 1 };
+        const double D1[3] = { 0.5, 0.5, 0.5 };
+        s->large_run = hydro_run_create(2, nb1, D1, 0.1, 1e-3, 1.0, r.max_it > kLargeRecord ? r.max_it : kLargeRecord,
+                                        HYDRO_FP64, nullptr);
+        if (!s->large_run) return HYDRO_ERR_HIP;
     }
-    const double C_min = r.C * ms;
-    double dt0 = 0;
-    int rc = hydro_ic(static_cast<double*>(p[0]), static_cast<double*>(p[1]), r.dim, r.nb, nullptr, r.nb, r.rho_in,
-                      r.rho_out, r.p_in, r.p_out, 1, r.rf, C_min, 1.4, &dt0, s->large_stream);
-    if (rc) return rc;
-    hydro_run* run = hydro_run_create(r.dim, r.nb, D, C_min, dt0, r.max_t, r.max_it, r.precision, nullptr);
-    if (!run) return HYDRO_ERR_HIP;
-    rc = hydro_run_advance(run, p[0], p[1], p[2], p[3], r.max_it, s->large_stream);
+    int rc = hydro::run_restart_blast(s->large_run, r.dim, r.nb, r.precision, r.C, r.rf, r.rho_in, r.rho_out, r.p_in,
+                                      r.p_out, r.max_t, r.max_it, p[0], p[1], s->large_stream);
+    if (rc == 0) rc = hydro_run_advance(s->large_run, p[0], p[1], p[2], p[3], r.max_it, s->large_stream);
     if (rc == 0 && hipEventRecord(s->large_ev, s->large_stream) != hipSuccess) {
         set_error("hydro server: cannot record the large solve's event");
         rc = HYDRO_ERR_HIP;
     }
     if (rc) {
         (void)hipStreamSynchronize(s->large_stream);
-        hydro_run_destroy(run);
         return rc;
     }
-    s->large_run = run;
     s->large_id = id;
     return 0;
 }
@@ -380,8 +386,6 @@ int retire_large(hydro_server* s, size_t* n_done)
     r.t_done = s->now();
     r.state = r.rc == 0 ? kDone : kFailed;
     s->cell_updates += static_cast<double>(r.it) * r.interior;
-    hydro_run_destroy(s->large_run);
-    s->large_run = nullptr;
     s->large_id = -1;
     if (!r.keep || r.state == kFailed) release_memory(s, r);
     if (n_done) ++*n_done;

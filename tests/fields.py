@@ -45,11 +45,13 @@ def field_errors(got, want):
     return out
 
 
-def own_scale_velocity_errors(got, want, floor: float = 1e-6):
+def own_scale_velocity_errors(got, want, floor: float = 1e-2):
     """{component: rel L-inf against the component's own maximum} for the velocity
-    components whose maximum in `want` is at least `floor` times the signal-speed scale;
-    a component below that holds only round-off (it vanishes by symmetry) and is left
-    out (None)."""
+    components whose maximum in `want` is at least `floor` times the signal-speed scale,
+    i.e. that carry a real part of the flow.  A component below that (one that vanishes by
+    symmetry, or has nearly died out, as across a 5 x 1 periodic channel after 60 steps)
+    is left out (None): against its own tiny maximum the FP32 round-off of the fluxes,
+    about 1e-8 of the signal speed, measures nothing about the flow."""
     a, b = primitives(*got), primitives(*want)
     vscale = signal_speed_scale(b)
     out = {}

@@ -224,10 +224,15 @@ def test_batch_fp32_flux_vs_fp64(gpu, dim):
         a = tuple(x.cpu().numpy() for x in ref.current(sr))
         b = tuple(x.cpu().numpy() for x in got.current(sg))
         errs = field_errors(b, a)
+        own = own_scale_velocity_errors(b, a)
         print(f"  {nb}: " + ", ".join(f"{k} {v[0]:.1e}" for k, v in errs.items())
-              + f" | max|v_ref| " + ", ".join(f"{np.max(np.abs(a[1][..., c])):.1e}" for c in range(dim)), flush=True)
+              + " | against own max " + ", ".join(f"{k} {v:.1e}" if v is not None else f"{k} -"
+                                                  for k, v in own.items()), flush=True)
         for k, (linf, _) in errs.items():
             worst[k] = max(worst.get(k, 0.0), linf)
+        for k, e in own.items():
+            if e is not None:
+                worst[k + "/own"] = max(worst.get(k + "/own", 0.0), e)
     print(f"[batch fp32-flux {dim}D] worst rel Linf per field: {worst}", flush=True)
     for k, v in worst.items():
         assert v <= 1e-5, (k, v)

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from fields import field_errors
+from fields import field_errors, own_scale_velocity_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -44,10 +44,14 @@ def test_fp32_flux_2d_vs_fp64(gpu, nb, steps):
     ref, td_ref, dm_ref = _run(gpu, nb, steps, gpu.FP64)
     got, td_got, dm_got = _run(gpu, nb, steps, gpu.FP32_FLUX)
     errs = field_errors(got, ref)
+    own = own_scale_velocity_errors(got, ref)
     print(f"[fp32-flux 2D {nb} x {steps}] " + ", ".join(f"{k}: Linf {v[0]:.2e} L2 {v[1]:.2e}" for k, v in errs.items())
+          + f"; velocities against their own max {own}"
           + f"; Dt rel {np.max(np.abs(td_got / td_ref - 1)):.2e}; mass drift {dm_got:.1e} (fp64 {dm_ref:.1e})")
     for k, (linf, _) in errs.items():
         assert linf <= TOL, (k, linf)
+    for k, e in own.items():
+        assert e is None or e <= TOL, (k, e)
     assert dm_got < 1e-13
 
 
@@ -56,8 +60,12 @@ def test_fp32_flux_3d_vs_fp64(gpu, nb, steps):
     ref, td_ref, dm_ref = _run(gpu, nb, steps, gpu.FP64, ic_mode=1, radius_frac=0.1)
     got, td_got, dm_got = _run(gpu, nb, steps, gpu.FP32_FLUX, ic_mode=1, radius_frac=0.1)
     errs = field_errors(got, ref)
+    own = own_scale_velocity_errors(got, ref)
     print(f"[fp32-flux 3D {nb} x {steps}] " + ", ".join(f"{k}: Linf {v[0]:.2e} L2 {v[1]:.2e}" for k, v in errs.items())
+          + f"; velocities against their own max {own}"
           + f"; Dt rel {np.max(np.abs(td_got / td_ref - 1)):.2e}; mass drift {dm_got:.1e}")
     for k, (linf, _) in errs.items():
         assert linf <= TOL, (k, linf)
+    for k, e in own.items():
+        assert e is None or e <= TOL, (k, e)
     assert dm_got < 1e-13

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 import torch
 
-from fields import field_errors
+from fields import field_errors, own_scale_velocity_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -99,11 +99,15 @@ def test_c3_fp32flux_vs_fp64_full_size(gpu):
     del got
     torch.cuda.empty_cache()
     errs = field_errors(got_np, ref_np)
+    own = own_scale_velocity_errors(got_np, ref_np)
     print("[c3 512^3 x 200, fp32-flux vs fp64] " + ", ".join(f"{k}: Linf {v[0]:.2e} L2 {v[1]:.2e}"
                                                            for k, v in errs.items())
+          + f"; velocities against their own max {own}"
           + f"; t {st_got['t']:.9e} vs {st_ref['t']:.9e}; mass drift {abs(g1 - g0) / g0:.1e}", flush=True)
     for k, (linf, _) in errs.items():
         assert linf <= 1e-5, (k, linf)
+    for k, e in own.items():
+        assert e is None or e <= 1e-5, (k, e)
     assert abs(g1 - g0) / float(np.prod(nb)) < 1e-13
 
 
