@@ -16,11 +16,19 @@
 //   * the Y face (j, j+1) is computed once by the wave of row j from the primitives
 //     the wave of row j+1 left in LDS, and left in LDS for the wave of row j+1
 //     (two workgroup barriers per X step, single-buffered LDS);
-//   * the X face between rows i and i+1 is carried in registers along the march.
+//   * the X face between rows i and i+1 is carried in registers along the march;
+//   * TY is 6 for FP64 and FP32 state and 10 for FP32-flux (kTY3);
+//   * workgroup b runs on XCD b % 8, and when the strips split evenly over the XCDs,
+//     XCD x owns the strips [x*S, (x+1)*S) and walks its tiles in groups of G, about
+//     the workgroups resident on the XCD at once, each group through all S strips
+//     before the next group: a strip's first rows are then read just after the
+//     previous strip's last rows by the same tiles, on the same L2 (512^3: 0.5-1%
+//     faster than strip-major order over the whole chip).
 // rhoE is read and written as 16-byte (rho, E) pairs and uvw as 24-byte triples; lanes
 // are consecutive in Z, so every wave row is one contiguous stream per array.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <type_traits>
 
@@ -41,15 +49,23 @@ struct Geom3 {
     int nzw, nyw;      // Z windows, Y tiles
     int wz;            // owned Z columns per window (z_window_width)
     int wrap_x, wrap_y, wrap_z;  // periodic axes read from the opposite interior plane
+    int xcd_strips, group;       // strips per XCD, tile group (XCD-major order; 0: strip-major)
 };
 
 // TS: state and update precision; TF, M: precision and math policy of the primitives,
 // fluxes and signal speeds (see the 2D kernel).
-template <typename TS>
-constexpr int kMinWaves3 = std::is_same_v<TS, float> ? HYDRO_3D_MIN_WAVES_FP32 : HYDRO_3D_MIN_WAVES;
+template <typename TS, typename TF>
+constexpr bool kFlux3 = std::is_same_v<TS, double> && std::is_same_v<TF, float>;
+template <typename TS, typename TF>
+constexpr int kTY3 = kFlux3<TS, TF> ? kTYFlux : kTY;  // owned Y rows per workgroup
+// Minimum waves per SIMD: 4 for FP64 (two 8-wave workgroups per CU, at most 128 VGPRs),
+// 3 for FP32-flux (one 12-wave workgroup), HYDRO_3D_MIN_WAVES_FP32 for FP32 state.
+template <typename TS, typename TF>
+constexpr int kMinWaves3 = std::is_same_v<TS, float> ? HYDRO_3D_MIN_WAVES_FP32
+                                                     : (kFlux3<TS, TF> ? 3 : HYDRO_3D_MIN_WAVES);
 
 template <typename TS, typename TF, typename M, bool WALL_Y, bool WALL_Z, bool RESIDENT>
-__global__ void __launch_bounds__(kBlock3, (kMinWaves3<TS>))
+__global__ void __launch_bounds__((kTY3<TS, TF> + 2) * kWave, (kMinWaves3<TS, TF>))
     fvm3d_kernel(const TS* __restrict__ rhoE, const TS* __restrict__ uvw, TS* __restrict__ rhoE_next,
                  TS* __restrict__ uvw_next, Geom3 g, Consts3<TS> cs, TS DtDx_arg, TS DtDy_arg, TS DtDz_arg,
                  StepReduce* red, StepControl* ctl, TS* Dt_next, double* traj_t, TS* traj_dt)
@@ -72,18 +88,41 @@ __global__ void __launch_bounds__(kBlock3, (kMinWaves3<TS>))
     Trip<TS>* __restrict__ MN3 = reinterpret_cast<Trip<TS>*>(uvw_next);
 
     const int lane = threadIdx.x & (kWave - 1);
-    __shared__ int s_simd[kNW];
+    constexpr int TY = kTY3<TS, TF>, NW = TY + 2;
+    __shared__ int s_simd[NW];
     const int w = tile_row_of_wave(s_simd);  // tile row of this wave (halo rows placed by SIMD)
-    const unsigned lb = xcd_remap3(blockIdx.x, gridDim.x);
     const unsigned per_strip = static_cast<unsigned>(g.nzw) * static_cast<unsigned>(g.nyw);
-    const int strip = static_cast<int>(lb / per_strip);
-    const unsigned rem = lb % per_strip;
+    int strip;
+    unsigned rem;
+    if (g.xcd_strips > 0) {
+        // XCD-major order (file comment): XCD x = b % 8 takes its u-th workgroup as
+        // (tile group, strip, tile in group), the last group holding the remainder.
+        const unsigned S = static_cast<unsigned>(g.xcd_strips), G = static_cast<unsigned>(g.group);
+        const unsigned x = blockIdx.x % 8u, u = blockIdx.x / 8u;
+        const unsigned full = per_strip / G, in_full = full * G * S;
+        unsigned s_in, tile;
+        if (u < in_full) {
+            const unsigned gi = u / (G * S), r = u % (G * S);
+            s_in = r / G;
+            tile = gi * G + r % G;
+        } else {
+            const unsigned Gl = per_strip - full * G, r = u - in_full;
+            s_in = r / Gl;
+            tile = full * G + r % Gl;
+        }
+        strip = static_cast<int>(x * S + s_in);
+        rem = tile;
+    } else {
+        const unsigned lb = xcd_remap3(blockIdx.x, gridDim.x);
+        strip = static_cast<int>(lb / per_strip);
+        rem = lb % per_strip;
+    }
     const int yt = static_cast<int>(rem / static_cast<unsigned>(g.nzw));
     const int zw = static_cast<int>(rem % static_cast<unsigned>(g.nzw));
 
-    const int j = yt * kTY + w;  // row of this wave (j0 - 1 + w with j0 = 1 + yt*TY)
+    const int j = yt * TY + w;  // row of this wave (j0 - 1 + w with j0 = 1 + yt*TY)
     const int k = zw * g.wz + lane;
-    const bool halo_wave = (w == 0) || (w == kNW - 1);
+    const bool halo_wave = (w == 0) || (w == NW - 1);
     const bool owned = !halo_wave && j <= g.nb_y && lane >= 1 && lane <= g.wz && k <= g.nb_z;
     // Reflective walls as in 2D: a ghost row (wave) / ghost column (lane) loads its
     // interior neighbour and presents its mirror, normal velocity negated, to the face
@@ -108,7 +147,7 @@ __global__ void __launch_bounds__(kBlock3, (kMinWaves3<TS>))
     const int i0 = 1 + strip * g.rows;
     const int i1 = min(i0 + g.rows - 1, g.nb_x);
 
-    __shared__ March3Lds<TF> lds;
+    __shared__ March3Lds<TF, NW> lds;
     const uint64_t key = march3d<TS, TF, M>(R2, M3, RN2, MN3, g.zy, col, g.nb_x, i0, i1, w, lane, halo_wave, owned, ymask,
                                             zmask, g.wrap_x != 0, gamma, gmo, divgamma, K, DtDx, DtDy, DtDz, lds, 0);
     block_max_to_slot(red, key);
@@ -182,14 +221,26 @@ __global__ void ghost_z3d_kernel(T* __restrict__ rhoE, T* __restrict__ uvw, long
 }
 
 // Maximum X rows per strip (no effect on results), set by hydro_tune_set; 0 selects the
-// per-precision default: 32 rows, 16 for FP32-flux (sweeps of 4..64 rows on the 512^3
-// blast: FP64 is fastest at 32, FP32-flux 3% faster at 8..16 than at 32).  The rows of
-// the last launch (after the strip adaptation) are kept for reporting.
+// default of 32 rows (512^3 blast: FP64 and FP32 state fastest at 32 of 4..64 rows;
+// FP32-flux at its 10-row tiles 2.34 ms at 32 rows, 2.41 ms at 16).  The rows of the
+// last launch (after the strip adaptation) are kept for reporting.
 std::atomic<int> g_rows_3d{0};
 std::atomic<int> g_last_rows_3d{0};
 // Workgroups a step should have to fill the chip (4 per CU); smaller grids use shorter
 // strips (down to 4 rows) until they have that many.
 constexpr long long kFillBlocks3 = 1024;
+
+// Workgroups of one 3D step kernel resident on one XCD at once (the tile group of the
+// XCD-major order), from the kernel's occupancy; 0 if it cannot be queried.
+template <typename KernelFn>
+int resident_per_xcd(KernelFn kernel, int block)
+{
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess) return 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return per_cu > 0 && cus >= 8 ? per_cu * (cus / 8) : 0;
+}
 
 template <typename T, typename TF = T, typename M = IeeeMath>
 int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
@@ -218,20 +269,36 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
     g.zy = (static_cast<long long>(nb_y) + 2) * g.zs;
     g.nzw = z_windows(g.nb_z);
     g.wz = z_window_width(g.nb_z);
-    g.nyw = static_cast<int>((nb_y + kTY - 1) / kTY);
+    constexpr int TY = kTY3<T, TF>;
+    g.nyw = static_cast<int>((nb_y + TY - 1) / TY);
     g.wrap_x = (wraps & kWrapX) ? 1 : 0;
     g.wrap_y = (wraps & kWrapY) ? 1 : 0;
     g.wrap_z = (wraps & kWrapZ) ? 1 : 0;
     const int rows_set = g_rows_3d.load(std::memory_order_relaxed);
-    g.rows = rows_set > 0 ? rows_set : (std::is_same_v<T, double> && std::is_same_v<TF, float> ? 16 : 32);
+    g.rows = rows_set > 0 ? rows_set : 32;
     while (g.rows > 4 && (static_cast<long long>(nb_x) + g.rows - 1) / g.rows * g.nzw * g.nyw < kFillBlocks3) g.rows /= 2;
     const long long nstrips = (static_cast<long long>(nb_x) + g.rows - 1) / g.rows;
     const long long nblocks = nstrips * g.nzw * g.nyw;
+    g.xcd_strips = 0;
+    g.group = 0;
+    if (nstrips % 8 == 0) {
+        // One query per kernel instantiation (all wall variants share its occupancy).
+        static std::atomic<int> resident{ -1 };
+        int r = resident.load(std::memory_order_relaxed);
+        if (r < 0) {
+            r = resident_per_xcd(fvm3d_kernel<T, TF, M, true, true, true>, (TY + 2) * kWave);
+            resident.store(r, std::memory_order_relaxed);
+        }
+        if (r > 0) {
+            g.xcd_strips = static_cast<int>(nstrips / 8);
+            g.group = static_cast<int>(std::min<long long>(r, static_cast<long long>(g.nzw) * g.nyw));
+        }
+    }
     if (nblocks > 0x7fffffffLL) {
         set_error("hydro 3D step: grid too large (%lld workgroups)", nblocks);
         return HYDRO_ERR_ARG;
     }
-    const dim3 grid(static_cast<unsigned>(nblocks)), block(kBlock3);
+    const dim3 grid(static_cast<unsigned>(nblocks)), block((TY + 2) * kWave);
     const bool wy = bc_y == HYDRO_BC_REFLECTIVE, wz = bc_z == HYDRO_BC_REFLECTIVE;
     g_last_rows_3d.store(g.rows, std::memory_order_relaxed);
     if (ev_a) HYDRO_HIP_OR(hipEventRecord(ev_a, s), HYDRO_ERR_HIP);

@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(kBlock3, 4)
                    const unsigned* __restrict__ seg_off, Consts<double> cs, unsigned nworkers)
 {
     using V2 = typename Vec2_3<TS>::type;
-    __shared__ March3Lds<TF> lds;
+    __shared__ March3Lds<TF, kNW> lds;
     const unsigned wid = xcd_remap3(blockIdx.x, gridDim.x);
     if (wid >= nworkers) return;  // workgroup-uniform
     const int lane = threadIdx.x & (kWave - 1);

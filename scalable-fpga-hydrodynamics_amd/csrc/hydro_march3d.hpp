@@ -1,6 +1,6 @@
 // The 3D block march shared by the single-domain step kernel and the batched kernel.
 //
-// A workgroup of kNW = TY + 2 waves covers a (Y x Z) tile of TY rows x wz columns and
+// A workgroup of NW = TY + 2 waves covers a (Y x Z) tile of TY rows x wz columns and
 // marches along X over rows i0..i1 (reading i0-1 .. i1+1); wave w holds Y row
 // j0 - 1 + w (waves 0 and TY+1 are halo rows: they load and convert their row but
 // update nothing) and lane l holds Z column k:
@@ -36,6 +36,13 @@ __host__ __device__ __forceinline__ int z_window_width(int nb_z)
 #ifndef HYDRO_3D_TY
 #define HYDRO_3D_TY 6
 #endif
+// Owned Y rows per workgroup of the plain FP32-flux step: its FP32 arithmetic leaves the
+// registers for 12-wave workgroups, one per CU, and the taller tile re-reads fewer halo
+// rows (512^3: 2.33 vs 2.45 ms per step at 6 rows).  FP64 at 10 rows has too few waves
+// per CU to hide its arithmetic (2.97 vs 2.67 ms).
+#ifndef HYDRO_3D_TY_FLUX
+#define HYDRO_3D_TY_FLUX 10
+#endif
 // Rows in flight ahead of row i+1 (register prefetch ring depth).
 #ifndef HYDRO_3D_PREFETCH
 #define HYDRO_3D_PREFETCH 1
@@ -49,9 +56,10 @@ __host__ __device__ __forceinline__ int z_window_width(int nb_z)
 #ifndef HYDRO_3D_MIN_WAVES_FP32
 #define HYDRO_3D_MIN_WAVES_FP32 8
 #endif
-constexpr int kTY = HYDRO_3D_TY;     // owned Y rows per workgroup
+constexpr int kTY = HYDRO_3D_TY;           // owned Y rows per workgroup (FP64, FP32 state, batched)
+constexpr int kTYFlux = HYDRO_3D_TY_FLUX;  // owned Y rows per workgroup of the plain FP32-flux step
 constexpr int kPF3 = HYDRO_3D_PREFETCH;
-constexpr int kNW = kTY + 2;         // waves per workgroup
+constexpr int kNW = kTY + 2;               // waves per workgroup at kTY
 constexpr int kBlock3 = kNW * kWave;
 
 template <typename T>
@@ -83,14 +91,15 @@ __device__ __forceinline__ unsigned xcd_remap3(unsigned b, unsigned n)
     return xcd * q + (xcd < r ? xcd : r) + k;
 }
 
-// Tile row held by the calling wave (0 and kNW-1 are the halo rows).  The waves of a
-// workgroup sit two per SIMD, and a halo wave does a fraction of an owned row's work.
+// Tile row held by the calling wave (0 and NW-1 are the halo rows).  The waves of a
+// workgroup sit two or more per SIMD, and a halo wave does a fraction of an owned row's work.
 // The halo rows go to waves on SIMDs 0 and 1 in even workgroup slots of the CU and on
 // SIMDs 2 and 3 in odd ones (HW_ID register), so that co-resident workgroups spread the
 // owned rows evenly over the four SIMDs.  Any outcome is a permutation of the rows, and
 // the march's results do not depend on which wave holds which row.  Every thread of
 // the workgroup must call it (one barrier).
-__device__ __forceinline__ int tile_row_of_wave(int (&s_simd)[kNW])
+template <int NW>
+__device__ __forceinline__ int tile_row_of_wave(int (&s_simd)[NW])
 {
     const int hw = static_cast<int>(threadIdx.x) / kWave;
     const unsigned id = __builtin_amdgcn_s_getreg((31 << 11) | 4);  // HW_REG_HW_ID, bits 31:0
@@ -99,13 +108,13 @@ __device__ __forceinline__ int tile_row_of_wave(int (&s_simd)[kNW])
     __syncthreads();
     const int want0 = (slot & 1) * 2, want1 = want0 + 1;
     int lo = 0;  // first wave on SIMD want0 (wave 0 if none)
-    for (int v = kNW - 1; v >= 0; --v)
+    for (int v = NW - 1; v >= 0; --v)
         if (s_simd[v] == want0) lo = v;
-    int hi = lo == kNW - 1 ? kNW - 2 : kNW - 1;  // first other wave on SIMD want1
-    for (int v = kNW - 1; v >= 0; --v)
+    int hi = lo == NW - 1 ? NW - 2 : NW - 1;  // first other wave on SIMD want1
+    for (int v = NW - 1; v >= 0; --v)
         if (v != lo && s_simd[v] == want1) hi = v;
     if (hw == lo) return 0;
-    if (hw == hi) return kNW - 1;
+    if (hw == hi) return NW - 1;
     return 1 + hw - (lo < hw ? 1 : 0) - (hi < hw ? 1 : 0);
 }
 
@@ -156,22 +165,22 @@ __device__ __forceinline__ long long row3_of(int r, int nb_x, bool wrap_x)
     return wrap_x ? (r == 0 ? nb_x : (r == nb_x + 1 ? 1 : r)) : r;
 }
 
-template <typename TF>
+template <typename TF, int NW>
 struct March3Lds {
-    TF prim[kNW][6][kWave];  // Y-frame primitives (rho, v, u, w, p, a) of every wave
-    TF flux[kNW][5][kWave];  // Y-face fluxes of every wave
+    TF prim[NW][6][kWave];  // Y-frame primitives (rho, v, u, w, p, a) of every wave
+    TF flux[NW][5][kWave];  // Y-face fluxes of every wave
 };
 
 // March rows i0..i1 of one (Y x Z) tile; see the file comment.  `col` is the in-plane
 // offset jl * zs + kl of the column this lane loads; stores go to the same column.
-template <typename TS, typename TF, typename M>
+template <typename TS, typename TF, typename M, int NW>
 __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __restrict__ R2,
                                             const Trip<TS>* __restrict__ M3,
                                             typename Vec2_3<TS>::type* __restrict__ RN2,
                                             Trip<TS>* __restrict__ MN3, long long zy, long long col, int nb_x, int i0,
                                             int i1, int w, int lane, bool halo_wave, bool owned, unsigned int ymask,
                                             unsigned int zmask, bool wrap_x, TF gamma, TF gmo, TF divgamma, TF K,
-                                            TS DtDx, TS DtDy, TS DtDz, March3Lds<TF>& lds, uint64_t key)
+                                            TS DtDx, TS DtDy, TS DtDz, March3Lds<TF, NW>& lds, uint64_t key)
 {
     using V2 = typename Vec2_3<TS>::type;
     auto& s_prim = lds.prim;
@@ -214,7 +223,7 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
 
         // (b) Y face (j, j+1), hydro3d/kernel.cxx:239-278, evaluated by the wave of row j
         // from the primitives the wave of row j+1 published.
-        if (w < kNW - 1) {
+        if (w < NW - 1) {
             Prim3<TF> L, R;
             L.r = s_prim[w][0][lane];
             L.un = s_prim[w][1][lane];

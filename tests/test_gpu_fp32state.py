@@ -45,7 +45,7 @@ def test_launcher_f32_random_states_2d(gpu, oracle, nb):
     assert_bitwise(to_host(dtn), np.array([want_dt], F32), "Dt_next")
 
 
-@pytest.mark.parametrize("nb", [(4, 5, 7), (10, 6, 62), (9, 7, 63), (40, 8, 125), (33, 20, 30)])
+@pytest.mark.parametrize("nb", [(4, 5, 7), (10, 6, 62), (9, 7, 63), (40, 8, 125), (33, 20, 30), (64, 200, 130)])
 def test_launcher_f32_random_states_3d(gpu, oracle, nb):
     rhoE, uvw = random_state_3d(*nb, seed=nb[0] * 10000 + nb[1] * 100 + nb[2], dtype=F32)
     k = _k(gpu, nb, 0.4)

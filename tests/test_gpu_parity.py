@@ -15,7 +15,10 @@ pytestmark = pytest.mark.gpu
 SENTINEL = 12345.0
 
 SIZES_2D = [(1, 1), (5, 3), (17, 61), (33, 62), (20, 63), (9, 124), (130, 125), (70, 200), (129, 257)]
-SIZES_3D = [(3, 1, 1), (4, 5, 7), (10, 6, 62), (9, 7, 63), (12, 13, 64), (40, 8, 125), (33, 20, 30)]
+SIZES_3D = [(3, 1, 1), (4, 5, 7), (10, 6, 62), (9, 7, 63), (12, 13, 64), (40, 8, 125), (33, 20, 30),
+            # strips that split evenly over the 8 XCDs (XCD-major workgroup order), with a
+            # partial last tile group (64 x 200 x 130) and a group of the whole strip (32 x 14 x 9)
+            (64, 200, 130), (32, 14, 9)]
 
 
 def _consts(O, nb, C):
@@ -268,9 +271,12 @@ def test_launch_geometry_keeps_results_2d(gpu, precision):
 
 @pytest.mark.parametrize("precision", [64, 3264, 32])
 def test_launch_geometry_keeps_results_3d(gpu, precision):
-    nb = (300, 256, 256)
+    """Strip rows change the launch, never the result: 64-row strips (not shortened on
+    this grid, 8 strips, one per XCD), ragged 7-row strips (74 strips: strip-major order)
+    and 2-row strips (256 strips: XCD-major order), bitwise against the default."""
+    nb = (512, 256, 256)
     base, last = _geometry_run(gpu, nb, precision, 6, {"rows3d": 0})
-    assert last == {"rows3d": 16 if precision == 3264 else 32}
+    assert last == {"rows3d": 32}
     for rows in (64, 7, 2):
         got, last = _geometry_run(gpu, nb, precision, 6, {"rows3d": rows})
         assert last == {"rows3d": rows}
