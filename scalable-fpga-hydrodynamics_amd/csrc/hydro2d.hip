@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(kBlock, (kMinBlocks2<TS, TF>))
 
     uint64_t key = 0;
     if (win < g.nwin)
-        key = march2d<TS, TF, M>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned, g.wrap_x != 0, gamma, gmo,
-                                 divgamma, K, DtDx, DtDy, key);
+        key = march2d<TS, TF, M, false>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned, g.wrap_x != 0,
+                                        gamma, gmo, divgamma, K, DtDx, DtDy, key);
 
     block_max_to_slot(red, key);
 }

@@ -128,8 +128,11 @@ __global__ void __launch_bounds__(kBlock, kBatchWaves2<TF>)
         int jl;
         unsigned int ymask;
         window_lane2d<TF, true>(j, nb_y, false, jl, ymask);
-        key = march2d<TS, TF, M>(R2, M2, RN2, MN2, ys, nb_x, sg.i0, sg.i1, j, jl, ymask, owned, true, gamma, gmo,
-                                 divgamma, K, DtDx, DtDy, key);
+        // A segment with i0 > i1 marches its rows i1..i0 downward (odd tile rows of direct
+        // dispatch, so that vertically adjacent tiles read their shared halo rows together).
+        const bool rev = sg.i0 > sg.i1;
+        key = march2d_dir<TS, TF, M>(rev, R2, M2, RN2, MN2, ys, nb_x, rev ? sg.i1 : sg.i0, rev ? sg.i0 : sg.i1, j, jl,
+                                     ymask, owned, true, gamma, gmo, divgamma, K, DtDx, DtDy, key);
     }
     if (direct) {
         // Every wave of a direct workgroup reaches this point.  Its waves hold consecutive
@@ -403,6 +406,8 @@ int build_tables(hydro_batch* b)
                     sg.col = cols[ci].col;
                     sg.i0 = i0;
                     sg.i1 = std::min(i0 + R - 1, cols[ci].rows);
+                    // Direct dispatch: odd tile rows march downward (i0 > i1 in the table).
+                    if (direct && ((i0 - 1) / R) % 2 == 1) std::swap(sg.i0, sg.i1);
                     per[t % W].push_back(sg);
                     ++t;
                 }

@@ -16,7 +16,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <utility>
 
 #include "hydro_device.hpp"
 
@@ -37,9 +36,6 @@ constexpr int kWavesPerBlock = 4;
 #define HYDRO_2D_PREFETCH 3
 #endif
 constexpr int kPF = HYDRO_2D_PREFETCH;
-#ifndef HYDRO_2D_UNROLL
-#define HYDRO_2D_UNROLL 0
-#endif
 // Non-temporal (streaming) stores of the new state.
 #ifndef HYDRO_2D_NT_STORE
 #define HYDRO_2D_NT_STORE 0
@@ -131,9 +127,13 @@ __device__ __forceinline__ void window_lane2d(int j, int nb_y, bool wrap_y, int&
 }
 
 // One row of the march: the Y face of row i (lane j, handed to lane j+1), the primitives
-// of row i+1 (into Pn) and the X face (i, i+1) (into F2), the update of row i with the
-// X face (i-1, i) in F1, its store and the order key of its new signal speed.
-template <typename TS, typename TF, typename M>
+// of the next row n (into Pn) and the X face between rows i and n (into F2), the update
+// of row i with the X face carried from the previous row in F1, its store and the order
+// key of its new signal speed.  Forward (REV false): n = i+1, F1 is the face (i-1, i)
+// and F2 the face (i, i+1).  Reverse: n = i-1, F1 is the face (i, i+1) and F2 the face
+// (i-1, i).  Either way a face is evaluated as flux(lower row, upper row) and the update
+// takes (lower face - upper face), so both directions give the same bits.
+template <typename TS, typename TF, typename M, bool REV>
 __device__ __forceinline__ uint64_t row2d(const typename Vec2<TS>::type& rc, const typename Vec2<TS>::type& mc,
                                           const typename Vec2<TS>::type& rn, const typename Vec2<TS>::type& mn,
                                           const Prim2<TF>& Pc, Prim2<TF>& Pn, const TF (&F1)[4], TF (&F2)[4],
@@ -162,19 +162,22 @@ __device__ __forceinline__ uint64_t row2d(const typename Vec2<TS>::type& rc, con
     const TF fyL2 = from_prev_lane(fyR2);
     const TF fyL3 = from_prev_lane(fyR3);
 
-    // Row i+1 primitives and the X face (i, i+1).
+    // Next row's primitives and the X face between rows i and n.
     Pn.r = static_cast<TF>(rn.x);
     cons_to_prim2<TF, M>(Pn.r, static_cast<TF>(mn.x), static_cast<TF>(mn.y), static_cast<TF>(rn.y), gmo, gamma, Pn.un,
                          Pn.ut, Pn.p, Pn.a);
-    face_flux2<TF, M>(Pc, Pn, K, divgamma, F2[0], F2[1], F2[2], F2[3]);
+    if constexpr (REV) face_flux2<TF, M>(Pn, Pc, K, divgamma, F2[0], F2[1], F2[2], F2[3]);
+    else face_flux2<TF, M>(Pc, Pn, K, divgamma, F2[0], F2[1], F2[2], F2[3]);
+    const TF* FL = REV ? F2 : F1;  // face (i-1, i)
+    const TF* FU = REV ? F1 : F2;  // face (i, i+1)
 
     // Update, kernel.cxx:259-278: U' = (U + x) + y; Y fluxes are in the (rho, v, u, E)
     // frame, so rho*u takes component 2 and rho*v component 1.  Increments and update
     // are in the state precision TS.
-    const TS x_r = DtDx * (static_cast<TS>(F1[0]) - static_cast<TS>(F2[0]));
-    const TS x_u = DtDx * (static_cast<TS>(F1[1]) - static_cast<TS>(F2[1]));
-    const TS x_v = DtDx * (static_cast<TS>(F1[2]) - static_cast<TS>(F2[2]));
-    const TS x_E = DtDx * (static_cast<TS>(F1[3]) - static_cast<TS>(F2[3]));
+    const TS x_r = DtDx * (static_cast<TS>(FL[0]) - static_cast<TS>(FU[0]));
+    const TS x_u = DtDx * (static_cast<TS>(FL[1]) - static_cast<TS>(FU[1]));
+    const TS x_v = DtDx * (static_cast<TS>(FL[2]) - static_cast<TS>(FU[2]));
+    const TS x_E = DtDx * (static_cast<TS>(FL[3]) - static_cast<TS>(FU[3]));
     const TS y_r = DtDy * (static_cast<TS>(fyL0) - static_cast<TS>(fyR0));
     const TS y_u = DtDy * (static_cast<TS>(fyL2) - static_cast<TS>(fyR2));
     const TS y_v = DtDy * (static_cast<TS>(fyL1) - static_cast<TS>(fyR1));
@@ -195,23 +198,18 @@ __device__ __forceinline__ uint64_t row2d(const typename Vec2<TS>::type& rc, con
     return key;
 }
 
-template <typename F, int... P>
-__device__ __forceinline__ bool run_phases(F& step, std::integer_sequence<int, P...>)
-{
-    return (step(std::integral_constant<int, P>{}) && ...);
-}
-
-// March rows i0..i1 (i0 <= i1) of one window.  Lane columns: j (global, for stores and
-// ownership) and jl (the column loaded); wrap_x as in row_of.  Returns `key` folded with the order keys of
-// the new signal speeds of the owned cells.
+// March rows i0..i1 (i0 <= i1) of one window, from i0 up (REV false) or from i1 down.
+// Lane columns: j (global, for stores and ownership) and jl (the column loaded); wrap_x
+// as in row_of.  Returns `key` folded with the order keys of the new signal speeds of
+// the owned cells.  kPF rows of loads are in flight while a row is computed.
 //
-// kPF rows of loads are in flight while a row is computed.  HYDRO_2D_UNROLL: rows
-// i .. i+1+kPF live in a ring of kPF+2 register slots and the primitives and X-face
-// flux of rows i and i+1 in two slots each, and the loop is unrolled over the phases of
-// both rings (lcm(kPF+2, 2) rows), so that slot indices are compile-time constants and
-// advancing the march moves no registers (at the price of register pressure); otherwise
-// the rolled loop shifts the ring by register moves.
-template <typename TS, typename TF, typename M>
+// The batched form's direct dispatch alternates the direction by tile row, so that two
+// vertically adjacent tiles running side by side read their shared halo rows at the
+// same moment (both at their start or both at their end) rather than a tile's length
+// apart: 1.6-2% faster on the 256-blast mix.  The plain step marches forward only:
+// alternating its strips made it 1-4% slower (its halo rows already hit in cache, and
+// strips marching apart widen the band of rows in flight).
+template <typename TS, typename TF, typename M, bool REV>
 __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __restrict__ R2,
                                             const typename Vec2<TS>::type* __restrict__ M2,
                                             typename Vec2<TS>::type* __restrict__ RN2,
@@ -219,49 +217,17 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
                                             int i1, int j, int jl, unsigned int ymask, bool owned, bool wrap_x, TF gamma,
                                             TF gmo, TF divgamma, TF K, TS DtDx, TS DtDy, uint64_t key)
 {
-#if HYDRO_2D_UNROLL
     using V2 = typename Vec2<TS>::type;
-    constexpr int S = kPF + 2;                  // row slots: row r in slot (r - i0) % S
-    constexpr int U = (S % 2 == 0) ? S : 2 * S;  // phases of the row ring and the 2-slot rings
-    V2 ra, ma, qr[S], qm[S];
-    load_cell<TS>(R2, M2, row_of(i0 - 1, nb_x, wrap_x) * ys + jl, ra, ma);
-#pragma unroll
-    for (int q = 0; q <= kPF; ++q)
-        load_cell<TS>(R2, M2, row_of(min(i0 + q, i1 + 1), nb_x, wrap_x) * ys + jl, qr[q], qm[q]);
-    Prim2<TF> P[2];  // X frame: un = u, ut = v; P[(i - i0) % 2] is row i
-    TF F[2][4];      // F[(i - i0) % 2] is the X face (i-1, i)
-    {
-        Prim2<TF> Pm;
-        Pm.r = static_cast<TF>(ra.x);
-        cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma,
-                             Pm.un, Pm.ut, Pm.p, Pm.a);
-        P[0].r = static_cast<TF>(qr[0].x);
-        cons_to_prim2<TF, M>(P[0].r, static_cast<TF>(qm[0].x), static_cast<TF>(qm[0].y), static_cast<TF>(qr[0].y),
-                             gmo, gamma, P[0].un, P[0].ut, P[0].p, P[0].a);
-        face_flux2<TF, M>(Pm, P[0], K, divgamma, F[0][0], F[0][1], F[0][2], F[0][3]);
-    }
-    int i = i0;
-    auto step = [&](auto phase) -> bool {
-        constexpr int p = decltype(phase)::value;
-        constexpr int sc = p % S, sn = (p + 1) % S, sl = (p + kPF + 1) % S;
-        constexpr int pc = p % 2, pn = (p + 1) % 2;
-        // Prefetch row i+1+kPF (clamped to the last row this strip reads) into the slot
-        // row i-1 left.
-        load_cell<TS>(R2, M2, row_of(min(i + 1 + kPF, i1 + 1), nb_x, wrap_x) * ys + jl, qr[sl], qm[sl]);
-        key = row2d<TS, TF, M>(qr[sc], qm[sc], qr[sn], qm[sn], P[pc], P[pn], F[pc], F[pn], RN2, MN2, ys, i, j, ymask,
-                               owned, gamma, gmo, divgamma, K, DtDx, DtDy, key);
-        return ++i <= i1;
-    };
-    while (run_phases(step, std::make_integer_sequence<int, U>{})) {
-    }
-#else
-    using V2 = typename Vec2<TS>::type;
+    constexpr int d = REV ? -1 : 1;
+    const int first = REV ? i1 : i0, last = REV ? i0 : i1;
+    // The rows read are i0-1 .. i1+1; `clamp` keeps the prefetch inside them.
+    auto clamp = [&](int r) { return REV ? max(r, i0 - 1) : min(r, i1 + 1); };
     V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
-    load_cell<TS>(R2, M2, row_of(i0 - 1, nb_x, wrap_x) * ys + jl, ra, ma);
-    load_cell<TS>(R2, M2, row_of(i0, nb_x, wrap_x) * ys + jl, rc, mc);
+    load_cell<TS>(R2, M2, row_of(first - d, nb_x, wrap_x) * ys + jl, ra, ma);
+    load_cell<TS>(R2, M2, row_of(first, nb_x, wrap_x) * ys + jl, rc, mc);
 #pragma unroll
     for (int q = 0; q < kPF; ++q)
-        load_cell<TS>(R2, M2, row_of(min(i0 + 1 + q, i1 + 1), nb_x, wrap_x) * ys + jl, qr[q], qm[q]);
+        load_cell<TS>(R2, M2, row_of(clamp(first + d * (1 + q)), nb_x, wrap_x) * ys + jl, qr[q], qm[q]);
     Prim2<TF> Pm, Pc;  // X frame: un = u, ut = v
     Pm.r = static_cast<TF>(ra.x);
     cons_to_prim2<TF, M>(Pm.r, static_cast<TF>(ma.x), static_cast<TF>(ma.y), static_cast<TF>(ra.y), gmo, gamma, Pm.un,
@@ -269,15 +235,16 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
     Pc.r = static_cast<TF>(rc.x);
     cons_to_prim2<TF, M>(Pc.r, static_cast<TF>(mc.x), static_cast<TF>(mc.y), static_cast<TF>(rc.y), gmo, gamma, Pc.un,
                          Pc.ut, Pc.p, Pc.a);
-    TF F1[4];
-    face_flux2<TF, M>(Pm, Pc, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
-    for (int i = i0; i <= i1; ++i) {
-        // Prefetch row i+1+kPF (clamped to the last row this strip reads).
-        load_cell<TS>(R2, M2, row_of(min(i + 1 + kPF, i1 + 1), nb_x, wrap_x) * ys + jl, qr[kPF], qm[kPF]);
+    TF F1[4];  // the face between the first row and the row behind it
+    if constexpr (REV) face_flux2<TF, M>(Pc, Pm, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
+    else face_flux2<TF, M>(Pm, Pc, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
+    for (int i = first; REV ? i >= last : i <= last; i += d) {
+        // Prefetch the row kPF+1 ahead (clamped to the last row this strip reads).
+        load_cell<TS>(R2, M2, row_of(clamp(i + d * (1 + kPF)), nb_x, wrap_x) * ys + jl, qr[kPF], qm[kPF]);
         Prim2<TF> Pn;
         TF F2[4];
-        key = row2d<TS, TF, M>(rc, mc, qr[0], qm[0], Pc, Pn, F1, F2, RN2, MN2, ys, i, j, ymask, owned, gamma, gmo,
-                               divgamma, K, DtDx, DtDy, key);
+        key = row2d<TS, TF, M, REV>(rc, mc, qr[0], qm[0], Pc, Pn, F1, F2, RN2, MN2, ys, i, j, ymask, owned, gamma, gmo,
+                                    divgamma, K, DtDx, DtDy, key);
         // Advance the march.
         Pc = Pn;
         rc = qr[0];
@@ -290,8 +257,24 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
 #pragma unroll
         for (int q = 0; q < 4; ++q) F1[q] = F2[q];
     }
-#endif
     return key;
+}
+
+// march2d in the direction `rev` (wave-uniform).
+template <typename TS, typename TF, typename M>
+__device__ __forceinline__ uint64_t march2d_dir(bool rev, const typename Vec2<TS>::type* __restrict__ R2,
+                                                const typename Vec2<TS>::type* __restrict__ M2,
+                                                typename Vec2<TS>::type* __restrict__ RN2,
+                                                typename Vec2<TS>::type* __restrict__ MN2, long long ys, int nb_x,
+                                                int i0, int i1, int j, int jl, unsigned int ymask, bool owned,
+                                                bool wrap_x, TF gamma, TF gmo, TF divgamma, TF K, TS DtDx, TS DtDy,
+                                                uint64_t key)
+{
+    if (rev)
+        return march2d<TS, TF, M, true>(R2, M2, RN2, MN2, ys, nb_x, i0, i1, j, jl, ymask, owned, wrap_x, gamma, gmo,
+                                        divgamma, K, DtDx, DtDy, key);
+    return march2d<TS, TF, M, false>(R2, M2, RN2, MN2, ys, nb_x, i0, i1, j, jl, ymask, owned, wrap_x, gamma, gmo,
+                                     divgamma, K, DtDx, DtDy, key);
 }
 
 }  // namespace hydro
