@@ -1,7 +1,7 @@
 // The 2D strip march shared by the single-domain step kernel and the batched kernel.
 //
 // One wave (64 lanes, lane l <-> Y column j) marches along X over rows i0..i1 of its
-// column window, reading rows i0-1 .. i1+1:
+// column window, from i0 up or from i1 down, reading rows i0-1 .. i1+1:
 //   * the X face between rows i and i+1 is computed once and carried in registers to
 //     the next row (fx2 of row i is fx1 of row i+1; hydro2d/kernel.cxx:186-210 computes
 //     it twice with identical arguments);
@@ -10,12 +10,12 @@
 //   * primitives and the impedance sqrt(gamma*p*rho) are computed once per cell;
 //   * kPF rows of loads are kept in flight ahead of the row being converted.
 // The arithmetic of every cell is the reference's (hydro2d/kernel.cxx:132-319) in its
-// operation order, so results do not depend on how rows and windows are split.
+// operation order, so results do not depend on how rows and windows are split or on
+// the direction of the march.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
 
 #include "hydro_device.hpp"
 
