@@ -454,6 +454,7 @@ def bench_served(args) -> dict:
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
     srv.set_limits(args.serve_cap_cells, args.steps_per_tick)
     srv.set_policy(args.serve_large_ms / 1e3)
+    srv.set_backlog(args.serve_large_backlog)
     live = sv.run_trace(hydro, srv, reqs, realtime=True, progress=prog)
     srv.close()
     real_seq = None
@@ -466,6 +467,7 @@ def bench_served(args) -> dict:
     srv = hydro.Server(arena, capacity=256, steps_per_tick=args.steps_per_tick)
     srv.set_limits(args.serve_cap_cells, args.steps_per_tick)
     srv.set_policy(args.serve_large_ms / 1e3)
+    srv.set_backlog(args.serve_large_backlog)
     sat = sv.run_trace(hydro, srv, reqs, realtime=False, progress=prog)
     srv.close()
     seq_gcups = work / sum(dur) / 1e9
@@ -485,7 +487,7 @@ def bench_served(args) -> dict:
         "config": {"workload": "served_mixed", "requests": n, "mix": "70% 2D 128..4096, 30% 3D 64..256",
                    "steps": "100..2000", "precision": "50% fp32-flux / 50% fp64", "hbm_arena_gb": args.serve_hbm_gb,
                    "steps_per_tick": args.steps_per_tick, "engine_cap_cells": args.serve_cap_cells,
-                   "large_class_ms": args.serve_large_ms,
+                   "large_class_ms": args.serve_large_ms, "large_backlog": args.serve_large_backlog,
                    "arrival_rate_per_s": round(rate, 3),
                    "load": "0.8 of the one-at-a-time plain server"},
         "served": {"p50_s": round(live["p50"], 4), "p99_s": round(live["p99"], 4), "gcups": round(live["gcups"], 3),
@@ -532,7 +534,8 @@ def served_in_child(args) -> dict:
 
     cmd = [sys.executable, str(pathlib.Path(__file__).resolve()), "--workload", "served", "--requests",
            str(args.requests), "--serve-hbm-gb", str(args.serve_hbm_gb), "--steps-per-tick", str(args.steps_per_tick),
-           "--serve-cap-cells", str(args.serve_cap_cells), "--serve-large-ms", str(args.serve_large_ms)]
+           "--serve-cap-cells", str(args.serve_cap_cells), "--serve-large-ms", str(args.serve_large_ms),
+           "--serve-large-backlog", str(args.serve_large_backlog)]
     if args.no_real_sequential:
         cmd.append("--no-real-sequential")
     p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True, check=False)
@@ -593,6 +596,9 @@ def main() -> int:
     ap.add_argument("--serve-large-ms", type=float, default=20.0,
                     help="served workload: requests whose estimated solve time alone is at least this many ms run "
                          "one at a time in arrival order beside the smaller ones (0: one class)")
+    ap.add_argument("--serve-large-backlog", type=int, default=8,
+                    help="served workload: while more than this many large requests wait, the oldest join the "
+                         "batched engines (0: never)")
     ap.add_argument("--no-real-sequential", action="store_true",
                     help="served workload: skip the real one-at-a-time server run")
     ap.add_argument("--serve-cap-cells", type=float, default=6e6,
@@ -841,7 +847,8 @@ def main() -> int:
                              "sequential_plain_real": sv["sequential_plain_real"],
                              "sequential_sync_launcher": sv["sequential_sync_launcher"],
                              "by_dimension": sv["by_dimension"], "large_class": sv["large_class"],
-                             "large_class_ms": sv["config"]["large_class_ms"], "saturation": sv["saturation"]},
+                             "large_class_ms": sv["config"]["large_class_ms"],
+                             "large_backlog": sv["config"]["large_backlog"], "saturation": sv["saturation"]},
         }
     print(json.dumps(line), flush=True)
     return 0

@@ -359,6 +359,12 @@ int hydro_server_set_limits(hydro_server* server, double max_active_cells, size_
  * request in one class.  Applies to requests submitted after the call. */
 int hydro_server_set_policy(hydro_server* server, double large_seconds);
 
+/* Backlog of the large class: while more than large_backlog large requests wait, the
+ * oldest waiting ones are admitted to their batched engines as small ones are (under a
+ * long queue, one plain solve at a time is the slowest way through it).  0 (the
+ * default) keeps every large request in the one-at-a-time order. */
+int hydro_server_set_backlog(hydro_server* server, size_t large_backlog);
+
 /* ---------------------------------------------------------------------------------
  * Diagnostics.
  * ------------------------------------------------------------------------------- */

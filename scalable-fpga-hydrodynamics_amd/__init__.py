@@ -125,6 +125,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_server_stats.argtypes = [vp, vp, vp]
     L.hydro_server_set_limits.argtypes = [vp, d, sz]
     L.hydro_server_set_policy.argtypes = [vp, d]
+    L.hydro_server_set_backlog.argtypes = [vp, C.c_size_t]
     L.hydro_last_error.restype = C.c_char_p
     L.hydro_version.restype = i32
 
@@ -646,6 +647,13 @@ class Server:
         a time in arrival order as plain resident solves, beside the smaller ones in the
         batched engines (0: one class)."""
         _check(lib().hydro_server_set_policy(self._h, float(large_seconds)), "hydro_server_set_policy")
+
+    def set_backlog(self, large_backlog: int = 0) -> None:
+        """While more than large_backlog large requests wait, the oldest waiting ones join
+        their batched engines (0: never)."""
+        if int(large_backlog) < 0:
+            raise HydroError("Server.set_backlog: large_backlog >= 0")
+        _check(lib().hydro_server_set_backlog(self._h, int(large_backlog)), "hydro_server_set_backlog")
 
     def stats(self):
         t, cu = C.c_size_t(), C.c_double()

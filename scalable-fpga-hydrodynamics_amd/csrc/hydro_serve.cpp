@@ -25,7 +25,10 @@
 // once, beside it.  Sharing the device among everything admitted (the default, one
 // class) stretches the largest solves by the load, and a plain FIFO makes small solves
 // wait behind large ones; the size classes keep the large solves close to their FIFO
-// latency while the small ones overtake them.
+// latency while the small ones overtake them.  Under a backlog (hydro_server_set_backlog:
+// more than `large_backlog` large requests waiting), the oldest waiting large requests
+// join their batched engines as small ones do: one plain solve at a time is the slowest
+// way through a long queue of them.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -159,6 +162,7 @@ struct hydro_server {
     double active_cells[kEngines] = {};
     // Size classes: threshold (0: one class) and the one large request running, if any.
     double large_seconds = 0;
+    size_t large_backlog = 0;  // 0: large requests never join the batched engines
     int large_id = -1;
     hydro_run* large_run = nullptr;
     hipStream_t large_stream = nullptr;
@@ -282,18 +286,23 @@ int start_large(hydro_server* s, int id, size_t off)
     return 0;
 }
 
+size_t queued_large(const hydro_server* s)
+{
+    size_t n = 0;
+    for (const int id : s->queue) n += s->reqs[id].large ? 1 : 0;
+    return n;
+}
+
+// A waiting large request may join its batched engine: more than large_backlog wait.
+bool over_backlog(const hydro_server* s, size_t waiting) { return s->large_backlog > 0 && waiting > s->large_backlog; }
+
 int admit(hydro_server* s)
 {
+    size_t waiting = queued_large(s);
     for (auto q = s->queue.begin(); q != s->queue.end();) {
         const int id = *q;
         Req& r = s->reqs[id];
-        if (r.large) {
-            // One large request at a time, in arrival order (every later large request is
-            // held back too); small ones pass.
-            if (s->large_id >= 0) {
-                ++q;
-                continue;
-            }
+        if (r.large && s->large_id < 0) {
             size_t off = 0;
             if (!s->arena.alloc(r.bytes, off)) break;
             const int rc = start_large(s, id, off);
@@ -305,6 +314,13 @@ int admit(hydro_server* s)
             r.state = kRunning;
             r.t_admit = s->now();
             q = s->queue.erase(q);
+            --waiting;
+            continue;
+        }
+        if (r.large && !over_backlog(s, waiting)) {
+            // One large request at a time, in arrival order (every later large request is
+            // held back too); small ones pass.
+            ++q;
             continue;
         }
         const int e = engine_of(r);
@@ -338,6 +354,7 @@ int admit(hydro_server* s)
         ++s->running[e];
         s->active_cells[e] += r.interior;
         q = s->queue.erase(q);
+        if (r.large) --waiting;
     }
     return 0;
 }
@@ -551,9 +568,11 @@ try {
             any_busy = any_busy || !large_done;
             any_ready = any_ready || large_done;
         }
+        const bool backlog = over_backlog(s, queued_large(s));
         for (const int id : s->queue) {
             const Req& r = s->reqs[id];
-            any_ready = any_ready || (r.large ? s->large_id < 0 : !s->busy[engine_of(r)]);
+            const bool engine_free = !s->busy[engine_of(r)];
+            any_ready = any_ready || (r.large ? s->large_id < 0 || (backlog && engine_free) : engine_free);
         }
         if (any_ready || !any_busy) break;
         std::this_thread::sleep_for(std::chrono::microseconds(20));
@@ -676,6 +695,16 @@ try {
         return HYDRO_ERR_ARG;
     }
     s->large_seconds = large_seconds;  // classifies requests submitted from now on
+    return 0;
+} catch (...) {
+    hydro::set_exception_error(__func__);
+    return HYDRO_ERR_NOMEM;
+}
+
+extern "C" int hydro_server_set_backlog(hydro_server* s, size_t large_backlog)
+try {
+    if (!check(s, "hydro_server_set_backlog")) return HYDRO_ERR_ARG;
+    s->large_backlog = large_backlog;
     return 0;
 } catch (...) {
     hydro::set_exception_error(__func__);

@@ -47,13 +47,16 @@ def _solo_estimate(r):
     return r["steps"] * ((9e-6 + 13e-12 * cells) if len(r["nb"]) == 2 else (11e-6 + 24e-12 * cells))
 
 
-@pytest.mark.parametrize("large_s", [0.0, 9e-4])
-def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle, large_s):
-    """With one class, and with size classes (large requests run one at a time, in
-    arrival order, as plain resident solves beside the batched small ones)."""
+@pytest.mark.parametrize("large_s,backlog", [(0.0, 0), (9e-4, 0), (9e-4, 1)])
+def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle, large_s, backlog):
+    """With one class, with size classes (large requests run one at a time, in arrival
+    order, as plain resident solves beside the batched small ones), and with size
+    classes under a backlog of 1 (large requests waiting behind more than one other
+    join the batched engines)."""
     reqs = _mix()
     s = gpu.Server(1 << 30, capacity=16, steps_per_tick=13)
     s.set_policy(large_s)
+    s.set_backlog(backlog)
     ids = []
     # Staggered arrivals: a few requests, a tick, more requests (they join running work).
     for k, r in enumerate(reqs):
@@ -69,9 +72,11 @@ def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle, large_s):
     assert len(kinds) >= 3
     if large_s > 0:
         assert 2 <= len(large) < len(reqs)
-        spans = [(s.request(i)["admit"], s.request(i)["done"]) for i in large]
-        for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
-            assert d0 <= a1, (spans,)  # one large request at a time, in arrival order
+    spans = [(s.request(i)["admit"], s.request(i)["done"]) for i in large]
+    overlaps = sum(1 for (a0, d0), (a1, d1) in zip(spans, spans[1:]) if a1 < d0)
+    print(f"[serve] large spans overlapping their successor: {overlaps}", flush=True)
+    if large_s > 0 and backlog == 0:
+        assert overlaps == 0, (spans,)  # one large request at a time, in arrival order
     for r, rid in zip(reqs, ids):
         info = s.request(rid)
         assert info["rc"] == 0 and info["state"] == s.DONE and info["it"] == r["steps"], info
