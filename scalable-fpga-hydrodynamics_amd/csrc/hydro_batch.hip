@@ -70,6 +70,10 @@ namespace {
 
 // Waves per SIMD of the 2D batch kernel (= workgroups per CU): four, at most 128 VGPRs
 // (FP64 uses 128, FP32-flux 96; FP32-flux at five would spill).
+// Rows of loads in flight in the batched march: 2, one fewer than the plain step's.
+// Its tiles are 16 rows long; on the 256-blast mix 1 or 2 rows ran 0.5% faster than 3,
+// and 4 or 5 slower.
+constexpr int kPFBatch = 2;
 template <typename TF>
 constexpr int kBatchWaves2 = 4;
 
@@ -131,8 +135,9 @@ __global__ void __launch_bounds__(kBlock, kBatchWaves2<TF>)
         // A segment with i0 > i1 marches its rows i1..i0 downward (odd tile rows of direct
         // dispatch, so that vertically adjacent tiles read their shared halo rows together).
         const bool rev = sg.i0 > sg.i1;
-        key = march2d_dir<TS, TF, M>(rev, R2, M2, RN2, MN2, ys, nb_x, rev ? sg.i1 : sg.i0, rev ? sg.i0 : sg.i1, j, jl,
-                                     ymask, owned, true, gamma, gmo, divgamma, K, DtDx, DtDy, key);
+        key = march2d_dir<TS, TF, M, kPFBatch>(rev, R2, M2, RN2, MN2, ys, nb_x, rev ? sg.i1 : sg.i0,
+                                               rev ? sg.i0 : sg.i1, j, jl, ymask, owned, true, gamma, gmo, divgamma,
+                                               K, DtDx, DtDy, key);
     }
     if (direct) {
         // Every wave of a direct workgroup reaches this point.  Its waves hold consecutive

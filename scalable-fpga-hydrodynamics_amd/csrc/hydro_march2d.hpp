@@ -201,7 +201,7 @@ __device__ __forceinline__ uint64_t row2d(const typename Vec2<TS>::type& rc, con
 // March rows i0..i1 (i0 <= i1) of one window, from i0 up (REV false) or from i1 down.
 // Lane columns: j (global, for stores and ownership) and jl (the column loaded); wrap_x
 // as in row_of.  Returns `key` folded with the order keys of the new signal speeds of
-// the owned cells.  kPF rows of loads are in flight while a row is computed.
+// the owned cells.  PF rows of loads are in flight while a row is computed.
 //
 // The batched form's direct dispatch alternates the direction by tile row, so that two
 // vertically adjacent tiles running side by side read their shared halo rows at the
@@ -209,7 +209,7 @@ __device__ __forceinline__ uint64_t row2d(const typename Vec2<TS>::type& rc, con
 // apart: 1.6-2% faster on the 256-blast mix.  The plain step marches forward only:
 // alternating its strips made it 1-4% slower (its halo rows already hit in cache, and
 // strips marching apart widen the band of rows in flight).
-template <typename TS, typename TF, typename M, bool REV>
+template <typename TS, typename TF, typename M, bool REV, int PF = kPF>
 __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __restrict__ R2,
                                             const typename Vec2<TS>::type* __restrict__ M2,
                                             typename Vec2<TS>::type* __restrict__ RN2,
@@ -222,11 +222,11 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
     const int first = REV ? i1 : i0, last = REV ? i0 : i1;
     // The rows read are i0-1 .. i1+1; `clamp` keeps the prefetch inside them.
     auto clamp = [&](int r) { return REV ? max(r, i0 - 1) : min(r, i1 + 1); };
-    V2 ra, ma, rc, mc, qr[kPF + 1], qm[kPF + 1];
+    V2 ra, ma, rc, mc, qr[PF + 1], qm[PF + 1];
     load_cell<TS>(R2, M2, row_of(first - d, nb_x, wrap_x) * ys + jl, ra, ma);
     load_cell<TS>(R2, M2, row_of(first, nb_x, wrap_x) * ys + jl, rc, mc);
 #pragma unroll
-    for (int q = 0; q < kPF; ++q)
+    for (int q = 0; q < PF; ++q)
         load_cell<TS>(R2, M2, row_of(clamp(first + d * (1 + q)), nb_x, wrap_x) * ys + jl, qr[q], qm[q]);
     Prim2<TF> Pm, Pc;  // X frame: un = u, ut = v
     Pm.r = static_cast<TF>(ra.x);
@@ -239,8 +239,8 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
     if constexpr (REV) face_flux2<TF, M>(Pc, Pm, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
     else face_flux2<TF, M>(Pm, Pc, K, divgamma, F1[0], F1[1], F1[2], F1[3]);
     for (int i = first; REV ? i >= last : i <= last; i += d) {
-        // Prefetch the row kPF+1 ahead (clamped to the last row this strip reads).
-        load_cell<TS>(R2, M2, row_of(clamp(i + d * (1 + kPF)), nb_x, wrap_x) * ys + jl, qr[kPF], qm[kPF]);
+        // Prefetch the row PF+1 ahead (clamped to the last row this strip reads).
+        load_cell<TS>(R2, M2, row_of(clamp(i + d * (1 + PF)), nb_x, wrap_x) * ys + jl, qr[PF], qm[PF]);
         Prim2<TF> Pn;
         TF F2[4];
         key = row2d<TS, TF, M, REV>(rc, mc, qr[0], qm[0], Pc, Pn, F1, F2, RN2, MN2, ys, i, j, ymask, owned, gamma, gmo,
@@ -250,7 +250,7 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
         rc = qr[0];
         mc = qm[0];
 #pragma unroll
-        for (int q = 0; q < kPF; ++q) {
+        for (int q = 0; q < PF; ++q) {
             qr[q] = qr[q + 1];
             qm[q] = qm[q + 1];
         }
@@ -261,7 +261,7 @@ __device__ __forceinline__ uint64_t march2d(const typename Vec2<TS>::type* __res
 }
 
 // march2d in the direction `rev` (wave-uniform).
-template <typename TS, typename TF, typename M>
+template <typename TS, typename TF, typename M, int PF = kPF>
 __device__ __forceinline__ uint64_t march2d_dir(bool rev, const typename Vec2<TS>::type* __restrict__ R2,
                                                 const typename Vec2<TS>::type* __restrict__ M2,
                                                 typename Vec2<TS>::type* __restrict__ RN2,
@@ -271,9 +271,9 @@ __device__ __forceinline__ uint64_t march2d_dir(bool rev, const typename Vec2<TS
                                                 uint64_t key)
 {
     if (rev)
-        return march2d<TS, TF, M, true>(R2, M2, RN2, MN2, ys, nb_x, i0, i1, j, jl, ymask, owned, wrap_x, gamma, gmo,
+        return march2d<TS, TF, M, true, PF>(R2, M2, RN2, MN2, ys, nb_x, i0, i1, j, jl, ymask, owned, wrap_x, gamma, gmo,
                                         divgamma, K, DtDx, DtDy, key);
-    return march2d<TS, TF, M, false>(R2, M2, RN2, MN2, ys, nb_x, i0, i1, j, jl, ymask, owned, wrap_x, gamma, gmo,
+    return march2d<TS, TF, M, false, PF>(R2, M2, RN2, MN2, ys, nb_x, i0, i1, j, jl, ymask, owned, wrap_x, gamma, gmo,
                                      divgamma, K, DtDx, DtDy, key);
 }
 
