@@ -253,7 +253,7 @@ extern "C" int hydro_run_trajectory(hydro_run* r, size_t from, size_t count, dou
                                     void* stream)
 try {
     clear_error();
-    /* from > max_it first, so that max_it - from cannot wrap and from + count is never formed */
+    // from > max_it first, so that max_it - from cannot wrap and from + count is never formed.
     if (!r || from > r->max_it || count > r->max_it - from) {
         set_error("hydro_run_trajectory: range of %zu iterations from %zu outside the %zu recorded iterations",
                   count, from, r ? r->max_it : 0);

@@ -165,13 +165,18 @@ try {
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int dev = current_device();
     if (dev < 0) return HYDRO_ERR_HIP;
-    std::lock_guard<std::mutex> lk(g_state_mu);
-    const auto it = g_states.find(state_key(dev, s));
-    if (it == g_states.end()) return 0;  // no step ran on this stream
-    // The stream's steps must be done with the state before another stream takes it.
+    StepReduce* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_state_mu);
+        const auto it = g_states.find(state_key(dev, s));
+        if (it == g_states.end()) return 0;  // no step ran on this stream
+        p = it->second;
+        g_states.erase(it);  // unbound: no other stream can take it before it is returned
+    }
+    // The stream's steps must be done with the state before another stream takes it
+    // (waited for without the lock, so other threads' launches are not held up).
     HYDRO_HIP_OR(hipStreamSynchronize(s), HYDRO_ERR_HIP);
-    StepReduce* p = it->second;
-    g_states.erase(it);
+    std::lock_guard<std::mutex> lk(g_state_mu);
     StatePool& pool = g_pools[dev];
     if (in_pool(pool, p)) {
         pool.free.push_back(p);

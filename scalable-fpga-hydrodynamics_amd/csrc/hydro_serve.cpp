@@ -231,10 +231,6 @@ int ensure_engine(hydro_server* s, int e)
     return 0;
 }
 
-// Admit queued requests in arrival order.  A request whose engine is busy (mid-tick) or
-// full waits for it without holding back requests of other engines; admission stops at
-// the first request whose memory (or the active-cell cap) does not fit, so that later
-// requests never take the memory an earlier one is waiting for.
 // The four arrays of a request in its arena block: rhoE and momentum of buffer a, then
 // of buffer b.
 void buffers_of(hydro_server* s, const Req& r, size_t off, void* p[4])
@@ -296,6 +292,12 @@ size_t queued_large(const hydro_server* s)
 // A waiting large request may join its batched engine: more than large_backlog wait.
 bool over_backlog(const hydro_server* s, size_t waiting) { return s->large_backlog > 0 && waiting > s->large_backlog; }
 
+// Admit queued requests in arrival order.  A large request starts on the plain resident
+// run when no large request runs; otherwise it waits (later large requests wait behind
+// it) unless the backlog lets it join its engine.  A request whose engine is busy
+// (mid-tick), full or at its active-cell cap waits without holding back requests of other
+// engines.  Admission stops at the first request whose memory does not fit, so that later
+// requests never take the memory an earlier one is waiting for.
 int admit(hydro_server* s)
 {
     size_t waiting = queued_large(s);
