@@ -12,6 +12,9 @@ zero wall mass flux).  The same run at 4000 x 2000 must equal the oracle bit for
 the FP32-flux form against FP64 per field (relative L-inf <= 1e-5, scales as in
 fields.py), symmetry of the FP64 result in X, Y and Z, and 128^3 of the same IC equal to
 the oracle bit for bit.
+
+Both at full size also take their first steps bit for bit against the oracle run on the
+host (3 steps of 40000 x 20000, 5 of 512^3).
 """
 import numpy as np
 import pytest
@@ -84,6 +87,42 @@ def test_c2_reduced_matches_oracle(gpu, oracle):
     sl = (slice(1, -1),) * 2
     np.testing.assert_array_equal(cur[0].cpu().numpy()[sl].view(np.uint64), rhoE[sl].view(np.uint64))
     np.testing.assert_array_equal(cur[1].cpu().numpy()[sl].view(np.uint64), mom[sl].view(np.uint64))
+
+
+def _full_size_vs_oracle(gpu, oracle, nb, C, steps, ic_kw):
+    """The first steps at the full benchmark size, bit for bit against the oracle run on
+    the host's cores (the state is copied back once, at the end)."""
+    import time
+
+    t0 = time.perf_counter()
+    cur, st, _, _ = _run(gpu, nb, C, steps, gpu.FP64, **ic_kw)
+    print(f"[full {nb}] GPU {steps} steps done, t={st['t']:.9e}", flush=True)
+    got_r = cur[0].cpu().numpy()
+    got_m = cur[1].cpu().numpy()
+    del cur
+    torch.cuda.empty_cache()
+    dim = len(nb)
+    if dim == 2:
+        rhoE, mom, dt0 = oracle.ic2d(*nb)
+        res = oracle.run2d(rhoE, mom, dt0, 1e30, steps, nthreads=16)
+    else:
+        rhoE, mom, dt0 = oracle.ic3d(*nb, **ic_kw)
+        res = oracle.run3d(rhoE, mom, dt0, 1e30, steps, nthreads=16)
+    print(f"[full {nb}] oracle done after {time.perf_counter() - t0:.1f} s", flush=True)
+    assert res["it"] == st["it"] and res["t"] == st["t"] and res["dt"] == st["dt"]
+    sl = (slice(1, -1),) * dim
+    np.testing.assert_array_equal(got_r[sl].view(np.uint64), rhoE[sl].view(np.uint64))
+    np.testing.assert_array_equal(got_m[sl].view(np.uint64), mom[sl].view(np.uint64))
+
+
+def test_c2_full_size_first_steps_match_oracle(gpu, oracle):
+    """40000 x 20000, the headline workload itself: its first 3 steps equal the oracle's."""
+    _full_size_vs_oracle(gpu, oracle, (40000, 20000), 0.5, 3, {})
+
+
+def test_c3_full_size_first_steps_match_oracle(gpu, oracle):
+    """512^3 with the parameterised IC: its first 5 steps equal the oracle's."""
+    _full_size_vs_oracle(gpu, oracle, (512, 512, 512), 0.4, 5, {"ic_mode": 1, "radius_frac": 0.1})
 
 
 def test_c3_fp32flux_vs_fp64_full_size(gpu):
