@@ -130,3 +130,40 @@ def test_resident_run_f32(gpu, oracle, dim, nb, bc, steps):
     inner = interior2 if dim == 2 else interior3
     assert_bitwise(inner(got[0]), inner(rhoE), "rhoE after run")
     assert_bitwise(inner(got[1]), inner(mom), "momentum after run")
+
+
+@pytest.mark.parametrize("dim,nb,steps", [(2, (40000, 20000), 3), (3, (512, 512, 512), 5)])
+def test_resident_run_f32_full_size_first_steps(gpu, oracle, dim, nb, steps):
+    """The FP32-state benchmark sizes themselves (40000 x 20000 with the reference IC,
+    512^3 with the parameterised one): the first steps, every interior value and each
+    step's t and Dt, equal the oracle's FP32 loop."""
+    ic_kw = {} if dim == 2 else {"ic_mode": 1, "radius_frac": 0.1}
+    if dim == 2:
+        rhoE, mom, dt0 = oracle.ic2d(*nb, dtype=F32)
+        k = _k(gpu, nb, 0.5)
+    else:
+        rhoE, mom, dt0 = oracle.ic3d(*nb, dtype=F32, **ic_kw)
+        k = _k(gpu, nb, 0.4)
+    a = [to_dev(rhoE), to_dev(mom)]
+    b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+    run = gpu.Run(list(nb), k["D"], k["C_min"], dt0, 2.0, steps, 32)
+    run.advance(a, b, steps)
+    st = run.status()
+    assert st["rc"] == 0, st
+    tt, td = run.trajectory(st["it"])
+    cur = a if st["current"] == 0 else b
+    got = (to_host(cur[0]), to_host(cur[1]))
+    run.close()
+    del a, b, cur
+    torch.cuda.empty_cache()
+    print(f"[fp32 full {nb}] GPU {steps} steps done", flush=True)
+    if dim == 2:
+        res = oracle.run2d(rhoE, mom, dt0, 2.0, steps, nthreads=16)
+    else:
+        res = oracle.run3d(rhoE, mom, dt0, 2.0, steps, nthreads=16)
+    assert st["it"] == res["it"] == steps
+    np.testing.assert_array_equal(tt, res["traj_t"])
+    np.testing.assert_array_equal(td, res["traj_dt"])
+    inner = interior2 if dim == 2 else interior3
+    assert_bitwise(inner(got[0]), inner(rhoE), "rhoE")
+    assert_bitwise(inner(got[1]), inner(mom), "momentum")
