@@ -656,6 +656,48 @@ double NAME(oracle_mass3d)(const real *rhoE, size_t nb_x, size_t nb_y, size_t nb
     return (double)(tm + cc);
 }
 
+/* The reference's subdomains and ranks (hydro2d/main.cxx:620-657, hydro3d/main.cxx:667-713):
+ * an sdx x sdy (x sdz) split, subdomain (a, b, c) with index s = (a*sdy + b)*sdz + c, of
+ * size NB/sd on each axis (the last one along an axis adding NB % sd) at base
+ * (NB/sd)*index; rank r owns the subdomains with s % ranks == r, in s order.  The mass
+ * (compute_total_mass, hydro2d/main.cxx:237-265, hydro3d/main.cxx:249-281) is then, per
+ * rank, one compensated long double sum over its subdomains' interior rho, each in
+ * i-major order, rounded to double; the ranks' doubles are added in rank order (what
+ * MPI_Allreduce(SUM) gives for one or two ranks). */
+static void sd_box(size_t nb, size_t sd, size_t idx, size_t *base, size_t *n)
+{
+    const size_t size = nb / sd;
+    *base = size * idx;
+    *n = size + (idx == sd - 1 ? nb % sd : 0);
+}
+
+double NAME(oracle_mass_sd)(const real *rhoE, int dim, const size_t *nb, const size_t *sd, size_t ranks)
+{
+    const size_t sdz = dim == 3 ? sd[2] : 1, nsd = sd[0] * sd[1] * sdz;
+    const size_t zs = dim == 3 ? nb[2] + 2 : 1, koff = dim == 3 ? 1 : 0;
+    double total = 0.0;
+    for (size_t r = 0; r < ranks; ++r) {
+        long double tm = 0.0L, cc = 0.0L;
+        for (size_t s = r; s < nsd; s += ranks) {
+            size_t b[3] = { 0, 0, 0 }, n[3] = { 1, 1, 1 };
+            sd_box(nb[0], sd[0], s / (sd[1] * sdz), &b[0], &n[0]);
+            sd_box(nb[1], sd[1], (s / sdz) % sd[1], &b[1], &n[1]);
+            if (dim == 3) sd_box(nb[2], sdz, s % sdz, &b[2], &n[2]);
+            for (size_t i = b[0] + 1; i <= b[0] + n[0]; ++i)
+                for (size_t j = b[1] + 1; j <= b[1] + n[1]; ++j)
+                    for (size_t k = b[2] + koff; k < b[2] + koff + n[2]; ++k) {
+                        const long double var = (long double)rhoE[2 * ((i * (nb[1] + 2) + j) * zs + k)];
+                        const long double t = tm + var;
+                        if (fabsl(tm) >= var) cc += (tm - t) + var;
+                        else cc += (var - t) + tm;
+                        tm = t;
+                    }
+        }
+        total += (double)(tm + cc);
+    }
+    return total;
+}
+
 /* ---------------------------------------------------------------------------------
  * The step loop of hydro2d/main.cxx:686-815 on one domain: ghost fill, DtDx = Dt/DX,
  * launcher, swap, Dt <- Dt_next, ++it, t += Dt, while (t < max_t && it < max_it).
@@ -753,6 +795,42 @@ int NAME(oracle_write_csv2d)(const char *path, const real *rhoE, const real *uv,
             const size_t n = (i + 1) * ys + (j + 1);
             fprintf(f, "%zu,%zu,%g,%g,%g,%g\n", i, j, (double)rhoE[2 * n], (double)rhoE[2 * n + 1], (double)uv[2 * n],
                     (double)uv[2 * n + 1]);
+        }
+    return fclose(f);
+}
+
+/* The CSV of write_results with subdomains and ranks (hydro2d/main.cxx:166-235,
+ * hydro3d/main.cxx:172-247): each rank's text is its subdomains' rows, in s order, each
+ * subdomain in i-major order with 0-based global indices, and the ranks' texts follow
+ * one another in rank order after the header (MPI_Exscan offsets). */
+int NAME(oracle_write_csv_sd)(const char *path, const real *rhoE, const real *mom, int dim, const size_t *nb,
+                              const size_t *sd, size_t ranks)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) return -1;
+    fputs(dim == 2 ? "x,y,rho,p,u,v\n" : "x,y,z,rho,p,u,v,w\n", f);
+    const size_t sdz = dim == 3 ? sd[2] : 1, nsd = sd[0] * sd[1] * sdz;
+    const size_t zs = dim == 3 ? nb[2] + 2 : 1;
+    for (size_t r = 0; r < ranks; ++r)
+        for (size_t s = r; s < nsd; s += ranks) {
+            size_t b[3] = { 0, 0, 0 }, n[3] = { 1, 1, 1 };
+            sd_box(nb[0], sd[0], s / (sd[1] * sdz), &b[0], &n[0]);
+            sd_box(nb[1], sd[1], (s / sdz) % sd[1], &b[1], &n[1]);
+            if (dim == 3) sd_box(nb[2], sdz, s % sdz, &b[2], &n[2]);
+            for (size_t i = b[0]; i < b[0] + n[0]; ++i)
+                for (size_t j = b[1]; j < b[1] + n[1]; ++j)
+                    for (size_t k = b[2]; k < b[2] + n[2]; ++k) {
+                        if (dim == 2) {
+                            const size_t c = (i + 1) * (nb[1] + 2) + (j + 1);
+                            fprintf(f, "%zu,%zu,%g,%g,%g,%g\n", i, j, (double)rhoE[2 * c], (double)rhoE[2 * c + 1],
+                                    (double)mom[2 * c], (double)mom[2 * c + 1]);
+                        } else {
+                            const size_t c = ((i + 1) * (nb[1] + 2) + (j + 1)) * zs + (k + 1);
+                            fprintf(f, "%zu,%zu,%zu,%g,%g,%g,%g,%g\n", i, j, k, (double)rhoE[2 * c],
+                                    (double)rhoE[2 * c + 1], (double)mom[3 * c], (double)mom[3 * c + 1],
+                                    (double)mom[3 * c + 2]);
+                        }
+                    }
         }
     return fclose(f);
 }

@@ -72,6 +72,10 @@ def _declare(L: C.CDLL) -> None:
         g("oracle_run3d").restype = sz
         g("oracle_write_csv2d").argtypes = [C.c_char_p, vp, vp, sz, sz]
         g("oracle_write_csv3d").argtypes = [C.c_char_p, vp, vp, sz, sz, sz]
+        szp = C.POINTER(C.c_size_t)
+        g("oracle_mass_sd").argtypes = [vp, i32, szp, szp, sz]
+        g("oracle_mass_sd").restype = C.c_double
+        g("oracle_write_csv_sd").argtypes = [C.c_char_p, vp, vp, i32, szp, szp, sz]
 
 
 def _fn(name: str, dtype) -> C._CFuncPtr:
@@ -202,6 +206,29 @@ def mass2d(rhoE) -> float:
 
 def mass3d(rhoE) -> float:
     return _fn("oracle_mass3d", rhoE.dtype)(_ptr(rhoE), *(s - 2 for s in rhoE.shape[:3]))
+
+
+def _sizes(v):
+    return (C.c_size_t * 3)(*(list(v) + [1] * (3 - len(v))))
+
+
+def mass_sd(rhoE, sd, ranks: int) -> float:
+    """compute_total_mass under an sd split and `ranks` MPI ranks (hydro2d/main.cxx:237-265,
+    hydro3d/main.cxx:249-281): per rank a compensated long double sum over its subdomains,
+    the ranks' doubles added in rank order."""
+    dim = rhoE.ndim - 1
+    nb = [s - 2 for s in rhoE.shape[:dim]]
+    return _fn("oracle_mass_sd", rhoE.dtype)(_ptr(rhoE), dim, _sizes(nb), _sizes(sd), int(ranks))
+
+
+def write_csv_sd(path, rhoE, mom, sd, ranks: int) -> None:
+    """write_results under an sd split and `ranks` MPI ranks (hydro2d/main.cxx:166-235)."""
+    dim = rhoE.ndim - 1
+    nb = [s - 2 for s in rhoE.shape[:dim]]
+    rc = _fn("oracle_write_csv_sd", rhoE.dtype)(str(path).encode(), _ptr(rhoE), _ptr(mom), dim, _sizes(nb),
+                                                 _sizes(sd), int(ranks))
+    if rc != 0:
+        raise OSError(f"oracle csv write failed: {path}")
 
 
 # ----------------------------------------------------------------------------- runs
