@@ -6,10 +6,12 @@
 // become one domain resident in HBM: the X ghost planes are filled on the device
 // (hydro_ghost_fill*), Dt is reduced on the device, and the step loop runs on the device
 // (hydro_run_*), with the host reading back only at write steps and at the end.
-// --sdx/--sdy/--sdz are accepted for compatibility; the result does not depend on them
-// (the reference's exact ghost exchange makes the split invisible).
+// --sdx/--sdy/--sdz do not change the solution (the reference's exact ghost exchange
+// makes the split invisible); with --ranks N they set, as in a run of the reference on N
+// MPI ranks, the order of the CSV rows (each rank's subdomains in turn) and how the
+// printed total mass is summed (per rank, then across ranks).
 //
-// Added options: --bcx/--bcy/--bcz periodic|reflective (reference: periodic X,
+// Added options: --ranks N (default 1), --bcx/--bcy/--bcz periodic|reflective (reference: periodic X,
 // reflective Y and Z), --precision fp64|fp32 (fp32 = the reference's USE_FLOAT build),
 // --sync-steps (one synchronous `launcher` call per step with the host reading Dt, the
 // reference's own loop structure), --no-output (no CSV files), --threads N (CSV
@@ -53,6 +55,7 @@ struct Options {
     bool sync_steps = false;
     bool no_output = false;
     int threads = 0;
+    size_t ranks = 1;  // MPI ranks of the reference run whose output order and mass sums to follow
 };
 
 double now_us()
@@ -108,6 +111,7 @@ int print_usage(const char* exec, const Options& o)
     printf(" --precision fp64|fp32  State precision (fp32 = the USE_FLOAT build). Default : fp64\n");
     printf(" --sync-steps  One synchronous launcher call per step, Dt read back by the host\n");
     printf(" --no-output  Do not write CSV files\n");
+    printf(" --ranks N  Order the CSV rows and sum the printed mass as a run on N MPI ranks. Default : 1\n");
     printf(" -h, --help Show this message and exit\n");
     return 0;
 }
@@ -176,25 +180,37 @@ struct HostCopy {
     }
 };
 
-// Compensated long double sum of interior rho, hydro2d/main.cxx:237-265.
+// compute_total_mass, hydro2d/main.cxx:237-265 (hydro3d/main.cxx:249-281): each rank a
+// compensated long double sum of the interior rho of its subdomains, in `order` (each box
+// i-major), rounded to double; the ranks' doubles added in rank order (MPI_Allreduce's
+// sum for one or two ranks).
 template <typename T>
-double total_mass(const T* rhoE, const size_t* nb)
+double total_mass(const T* rhoE, const size_t* nb, const std::vector<hydro_io::Box>& order)
 {
-    long double tm = 0.0L, c = 0.0L;
-    const size_t nz = kDim == 3 ? nb[2] : 1;
     const size_t zs = kDim == 3 ? nb[2] + 2 : 1;
+    const size_t koff = kDim == 3 ? 1 : 0;
     const size_t ys = nb[1] + 2;
-    for (size_t i = 1; i <= nb[0]; ++i)
-        for (size_t j = 1; j <= nb[1]; ++j)
-            for (size_t k = (kDim == 3 ? 1 : 0); k < (kDim == 3 ? nz + 1 : 1); ++k) {
-                const size_t n = (i * ys + j) * zs + k;
-                const long double var = static_cast<long double>(rhoE[2 * n]);
-                const long double t = tm + var;
-                if (fabsl(tm) >= var) c += (tm - t) + var;
-                else c += (var - t) + tm;
-                tm = t;
-            }
-    return static_cast<double>(tm + c);
+    double total = 0.0;
+    size_t b = 0;
+    while (b < order.size()) {
+        const size_t rank = order[b].rank;
+        long double tm = 0.0L, c = 0.0L;
+        for (; b < order.size() && order[b].rank == rank; ++b) {
+            const hydro_io::Box& x = order[b];
+            for (size_t i = x.base[0] + 1; i <= x.base[0] + x.n[0]; ++i)
+                for (size_t j = x.base[1] + 1; j <= x.base[1] + x.n[1]; ++j)
+                    for (size_t k = x.base[2] + koff; k < x.base[2] + koff + x.n[2]; ++k) {
+                        const size_t n = (i * ys + j) * zs + k;
+                        const long double var = static_cast<long double>(rhoE[2 * n]);
+                        const long double t = tm + var;
+                        if (fabsl(tm) >= var) c += (tm - t) + var;
+                        else c += (var - t) + tm;
+                        tm = t;
+                    }
+        }
+        total += static_cast<double>(tm + c);
+    }
+    return total;
 }
 
 struct Timers {
@@ -326,8 +342,10 @@ int run(const Options& o, const char* argv0)
         die_hip(hipMemcpy(host.re, d_re[cur], n_re * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
         die_hip(hipMemcpy(host.mo, d_mo[cur], n_mo * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     };
+    // The reference's subdomains in the order its output walks them, for --ranks N.
+    const std::vector<hydro_io::Box> order = hydro_io::reference_order(kDim, nb, o.sd, o.ranks);
     // The mass the reference prints, compute_total_mass (hydro2d/main.cxx:237-265): the
-    // host compensated long double sum of interior rho, from the (rho, E) array alone
+    // host compensated long double sums of interior rho, from the (rho, E) array alone
     // when no CSV needs the momenta (--no-output).
     auto host_mass = [&](int cur) {
         if (!o.no_output) {
@@ -337,7 +355,7 @@ int run(const Options& o, const char* argv0)
             host.alloc(n_re, 0);
             die_hip(hipMemcpy(host.re, d_re[cur], n_re * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy D2H");
         }
-        return total_mass<T>(host.re, nb);
+        return total_mass<T>(host.re, nb, order);
     };
     const double cells_i = static_cast<double>(nb[0]) * static_cast<double>(nb[1]) *
                            static_cast<double>(kDim == 3 ? nb[2] : 1);
@@ -350,8 +368,8 @@ int run(const Options& o, const char* argv0)
     auto write_csv = [&](size_t it, double t) {
         if (o.no_output) return;
         const std::string fname = o.out + "_" + std::to_string(it) + "_" + hydro_io::format_fixed4(t) + ".csv";
-        host.writer.start([fname, &host, &nb, &o] {
-            return hydro_io::write_csv<T>(fname, host.re, host.mo, kDim, nb, o.threads);
+        host.writer.start([fname, &host, &nb, &o, &order] {
+            return hydro_io::write_csv<T>(fname, host.re, host.mo, kDim, nb, o.threads, order);
         }, fname);
     };
 
@@ -557,7 +575,7 @@ int main(int argc, char* argv[])
 {
     Options o;
     if (kDim == 3) o.nb[1] = 200;
-    enum { OPT_BCX = 1000, OPT_BCY, OPT_BCZ, OPT_PREC, OPT_SYNC, OPT_NOOUT, OPT_THREADS };
+    enum { OPT_BCX = 1000, OPT_BCY, OPT_BCZ, OPT_PREC, OPT_SYNC, OPT_NOOUT, OPT_THREADS, OPT_RANKS };
     const char* short_options = kDim == 2 ? "j:k:x:y:t:i:o:w:h" : "j:k:l:x:y:z:t:i:o:w:h";
     const struct option long_options[] = { { "sdx", required_argument, nullptr, 'j' },
                                            { "sdy", required_argument, nullptr, 'k' },
@@ -576,6 +594,7 @@ int main(int argc, char* argv[])
                                            { "sync-steps", no_argument, nullptr, OPT_SYNC },
                                            { "no-output", no_argument, nullptr, OPT_NOOUT },
                                            { "threads", required_argument, nullptr, OPT_THREADS },
+                                           { "ranks", required_argument, nullptr, OPT_RANKS },
                                            { "help", no_argument, nullptr, 'h' },
                                            { nullptr, 0, nullptr, 0 } };
     int opt;
@@ -607,6 +626,15 @@ int main(int argc, char* argv[])
             case OPT_SYNC: o.sync_steps = true; break;
             case OPT_NOOUT: o.no_output = true; break;
             case OPT_THREADS: o.threads = std::stoi(optarg); break;
+            case OPT_RANKS: {
+                const int r = std::stoi(optarg);
+                if (r < 1) {
+                    fprintf(stderr, "Error: --ranks must be >= 1\n");
+                    return 1;
+                }
+                o.ranks = static_cast<size_t>(r);
+                break;
+            }
             default: fprintf(stderr, "Unknown option\n"); return 1;
             }
         }
@@ -614,10 +642,16 @@ int main(int argc, char* argv[])
         fprintf(stderr, "Bad option value: %s\n", e.what());
         return 1;
     }
-    for (int d = 0; d < kDim; ++d)
+    for (int d = 0; d < kDim; ++d) {
         if (o.nb[d] < 1) {
             fprintf(stderr, "Error: grid sizes must be >= 1\n");
             return 1;
         }
+        if (o.sd[d] < 1 || o.sd[d] > (size_t(1) << 30)) {
+            fprintf(stderr, "Error: the number of sub-domains along an axis must be in [1, 2^30]\n");
+            return 1;
+        }
+    }
+    if (kDim == 2) o.sd[2] = 1;
     return o.precision == 64 ? run<double>(o, argv[0]) : run<float>(o, argv[0]);
 }

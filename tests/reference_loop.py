@@ -14,7 +14,9 @@ import pathlib
 import numpy as np
 
 
-def run(O, nb, max_t, max_it, w, out_prefix, dtype=np.float64, bc=None):
+def run(O, nb, max_t, max_it, w, out_prefix, dtype=np.float64, bc=None, sd=None, ranks=1):
+    """sd, ranks: the subdomain split and MPI rank count whose CSV row order and mass sums
+    to follow (hydro2d/main.cxx:166-265); None: the whole domain on one rank."""
     dim = len(nb)
     C = 0.5 if dim == 2 else 0.4
     bc = list(bc) if bc is not None else [0] + [1] * (dim - 1)
@@ -23,8 +25,15 @@ def run(O, nb, max_t, max_it, w, out_prefix, dtype=np.float64, bc=None):
     else:
         rhoE, mom, dt = O.ic3d(*nb, dtype=dtype, C_cfl=C)
     k = O.constants(nb, C, dtype)
-    mass = O.mass2d if dim == 2 else O.mass3d
-    write = O.write_csv2d if dim == 2 else O.write_csv3d
+    if sd is None:
+        mass = O.mass2d if dim == 2 else O.mass3d
+        write = O.write_csv2d if dim == 2 else O.write_csv3d
+    else:
+        def mass(r):
+            return O.mass_sd(r, sd, ranks)
+
+        def write(path, r, m):
+            O.write_csv_sd(path, r, m, sd, ranks)
     m0 = mass(rhoE)
     lines = [f"Initial total mass       : {m0:.7e}", f"Initial Dt : {float(dt):.2e}"]
     files = []

@@ -54,7 +54,29 @@ CASES = [
     # file written in the background while the next steps run.
     ("hydro", (700, 400), 2.0, 12, 5, [], np.float64, None),
     ("hydro3d", (60, 50, 48), 2.0, 6, 4, [], np.float64, None),
+    # The reference on MPI ranks: CSV rows rank by rank, each rank's subdomains in turn,
+    # and the mass summed per rank (the first is its own small configuration, mpirun -np 2
+    # --sdx 2 --sdy 1).
+    ("hydro", (60, 60), 2.0, 200, 10, ["-j", "2", "-k", "1", "--ranks", "2"], np.float64, None),
+    ("hydro", (50, 40), 2.0, 60, 20, ["-j", "3", "-k", "2", "--ranks", "4"], np.float64, None),
+    ("hydro", (40, 30), 2.0, 30, 10, ["-j", "1", "-k", "3"], np.float64, None),
+    ("hydro3d", (20, 18, 16), 2.0, 20, 10, ["-j", "2", "-k", "2", "-l", "2", "--ranks", "3"], np.float64, None),
 ]
+
+
+def _split(extra, dim):
+    """(sd, ranks) of the driver arguments, or (None, 1) for the whole domain's order
+    (one rank with an X-only split walks the cells in that order)."""
+    sd = [2, 1, 1][:dim]
+    ranks = 1
+    for flag, val in zip(extra, extra[1:] + [""]):
+        if flag in ("-j", "-k", "-l"):
+            sd["jkl".index(flag[1])] = int(val)
+        if flag == "--ranks":
+            ranks = int(val)
+    if ranks == 1 and all(s == 1 for s in sd[1:]):
+        return None, 1
+    return tuple(sd), ranks
 
 
 @pytest.mark.parametrize("binary,nb,max_t,max_it,w,extra,dtype,bc", CASES)
@@ -64,7 +86,9 @@ def test_driver_matches_reference_loop(gpu, oracle, tmp_path, binary, nb, max_t,
     stdout = _run(binary, args, tmp_path)
     ref_dir = tmp_path / "ref"
     ref_dir.mkdir()
-    want_lines, want_files = reference_loop.run(oracle, nb, max_t, max_it, w, ref_dir / "gpu", dtype=dtype, bc=bc)
+    sd, ranks = _split(extra, len(nb))
+    want_lines, want_files = reference_loop.run(oracle, nb, max_t, max_it, w, ref_dir / "gpu", dtype=dtype, bc=bc,
+                                                sd=sd, ranks=ranks)
     got_lines = _driver_lines(stdout)
     assert got_lines == want_lines
     if "--no-output" in extra:
