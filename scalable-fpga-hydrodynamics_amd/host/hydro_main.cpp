@@ -6,10 +6,11 @@
 // become one domain resident in HBM: the X ghost planes are filled on the device
 // (hydro_ghost_fill*), Dt is reduced on the device, and the step loop runs on the device
 // (hydro_run_*), with the host reading back only at write steps and at the end.
-// --sdx/--sdy/--sdz do not change the solution (the reference's exact ghost exchange
-// makes the split invisible); with --ranks N they set, as in a run of the reference on N
-// MPI ranks, the order of the CSV rows (each rank's subdomains in turn) and how the
-// printed total mass is summed (per rank, then across ranks).
+// --sdx does not change the solution (the reference's exact ghost exchange makes an X
+// split invisible); with --ranks N it sets, as in a run of the reference on N MPI ranks,
+// the order of the CSV rows (each rank's subdomains in turn) and how the printed total
+// mass is summed (per rank, then across ranks).  --sdy/--sdz above 1 are refused: the
+// reference's kernel makes Y and Z subdomain edges walls, a different problem.
 //
 // Added options: --ranks N (default 1), --bcx/--bcy/--bcz periodic|reflective (reference: periodic X,
 // reflective Y and Z), --precision fp64|fp32 (fp32 = the reference's USE_FLOAT build),
@@ -651,6 +652,15 @@ int main(int argc, char* argv[])
             fprintf(stderr, "Error: the number of sub-domains along an axis must be in [1, 2^30]\n");
             return 1;
         }
+    }
+    // The reference's kernel puts its Y (and Z) walls at every subdomain's own edges
+    // (hydro2d/kernel.cxx:214, 235: col_pos == 1, col_pos == y_stride - 2), so a split
+    // along Y or Z solves separate channels coupled only through Dt; along X the ghost
+    // exchange makes a split exact.  One resident domain reproduces X splits only.
+    if (o.sd[1] != 1 || (kDim == 3 && o.sd[2] != 1)) {
+        fprintf(stderr, "Error: sub-domains along Y%s are not supported: the reference makes their edges walls; "
+                        "split along X only (-j)\n", kDim == 3 ? " or Z" : "");
+        return 1;
     }
     if (kDim == 2) o.sd[2] = 1;
     return o.precision == 64 ? run<double>(o, argv[0]) : run<float>(o, argv[0]);
