@@ -741,7 +741,8 @@ def main() -> int:
         nbytes = src.numel() * src.element_size()
         copy_cached_gbs = 2 * nbytes / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
         # The same copy with non-temporal loads and stores, the faster of the two on
-        # MI355X: the achievable ceiling quoted for a step that reads and writes once.
+        # MI355X (6.5 vs 6.4 TB/s): the achievable ceiling quoted for a step that reads
+        # and writes once.
         hydro.stream_copy(dst, src, nontemporal=True)
         torch.cuda.synchronize()
         c0.record()
@@ -811,8 +812,9 @@ def main() -> int:
         "cpu_baseline": cpu,
         "hbm_frac_whole_step": round(bpc * cells * K / elapsed / 1e9 / HBM_PEAK_GBS, 4),
         "achievable_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
-        "achievable_copy_basis": "the faster of the library's block-contiguous copy with cached and with "
-                                 "non-temporal loads and stores, over the state's buffer",
+        "achievable_copy_basis": "the faster of the library's stream copies (one 4 KiB chunk per 256-thread "
+                                 "block, XCD-contiguous) with cached and with non-temporal loads and stores, "
+                                 "over the state's buffer",
         "achievable_copy_cached_gbs": round(copy_cached_gbs, 1) if copy_cached_gbs else None,
         "achievable_copy_nt_gbs": round(copy_nt_gbs, 1) if copy_nt_gbs else None,
         "achievable_read_gbs": round(read_gbs, 1) if read_gbs else None,

@@ -1,5 +1,7 @@
 """The stream reference kernels (copy, non-temporal copy, read, fill) touch every byte exactly once, for
-sizes that end inside a block's chunk, on a chunk boundary, and below one block."""
+sizes that end inside a block's chunk, on a chunk boundary, and below one block.  The copies move 4 KiB
+(256 16-byte elements) per block, with the blocks renumbered per XCD: block counts that are and are not
+multiples of the 8 XCDs are included."""
 import pytest
 import torch
 
@@ -7,7 +9,8 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("nontemporal", [False, True])
-@pytest.mark.parametrize("n16", [1, 7, 16384, 16384 + 5, 3 * 16384, 5 * 16384 - 1, 1 << 20])
+@pytest.mark.parametrize("n16", [1, 7, 255, 256, 257, 8 * 256, 8 * 256 + 3, 9 * 256 - 1, 61 * 256 + 17, 16384,
+                                 16384 + 5, 3 * 16384, 5 * 16384 - 1, 1 << 20])
 def test_stream_copy_exact(gpu, n16, nontemporal):
     g = torch.Generator(device="cuda").manual_seed(n16)
     src = torch.randint(-(2**62), 2**62, (2 * n16 + 2,), dtype=torch.int64, device="cuda", generator=g)
