@@ -272,9 +272,10 @@ def test_launch_geometry_keeps_results_2d(gpu, precision):
 @pytest.mark.parametrize("precision", [64, 3264, 32])
 def test_launch_geometry_keeps_results_3d(gpu, precision):
     """Strip rows change the launch, never the result: 64-row strips (not shortened on
-    this grid, 8 strips, one per XCD), ragged 7-row strips (74 strips: strip-major order)
-    and 2-row strips (256 strips: XCD-major order), bitwise against the default."""
-    nb = (512, 256, 256)
+    this grid, 8 strips, one per XCD, with at least 1024 tiles in all), ragged 7-row strips
+    (74 strips: strip-major order) and 2-row strips (256 strips: XCD-major order), bitwise
+    against the default."""
+    nb = (512, 384, 256)
     base, last = _geometry_run(gpu, nb, precision, 6, {"rows3d": 0})
     assert last == {"rows3d": 32}
     for rows in (64, 7, 2):
