@@ -553,6 +553,104 @@ def served_in_child(args) -> dict:
     return json.loads(p.stdout.strip().splitlines()[-1])
 
 
+def _prims3(torch, rhoE, mom):
+    """rho, p, u, v, w of the interior cells (gamma = 1.4), float64."""
+    sl = (slice(1, -1),) * 3
+    rho = rhoE[sl + (0,)]
+    E = rhoE[sl + (1,)]
+    u, v, w = (mom[sl + (c,)] / rho for c in range(3))
+    p = 0.4 * (E - 0.5 * rho * (u * u + v * v + w * w))
+    return {"rho": rho, "p": p, "u": u, "v": v, "w": w}
+
+
+def bench_c3(args) -> dict:
+    """The 512^3 blast (sphere of radius 0.1 L, pressure ratio 100, periodic X, reflective
+    Y/Z) advanced 200 steps from the same IC in FP64 and in the FP32-flux form, each timed
+    on its own resident loop; the FP32-flux fields' per-field relative L-inf and L2 error
+    against the FP64 fields after those 200 steps, and both forms' mass and energy drift."""
+    import numpy as np
+    import torch
+
+    hydro = importlib.import_module("scalable-fpga-hydrodynamics_amd")
+    nb = (512, 512, 512)
+    steps = 200
+    cells = int(np.prod(nb))
+    k = hydro.constants(nb, 0.4)
+    shape = tuple(n + 2 for n in nb)
+    out = {}
+    for prec, tag in ((hydro.FP64, "fp64"), (hydro.FP32_FLUX, "fp32flux")):
+        a = [torch.zeros(shape + (2,), dtype=torch.float64, device="cuda"),
+             torch.zeros(shape + (3,), dtype=torch.float64, device="cuda")]
+        b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+        dt0 = hydro.ic(a[0], a[1], k["C_min"], ic_mode=1, radius_frac=0.1, p_in=10.0, p_out=0.1)
+        m0, e0 = hydro.total_mass(a[0]), hydro.total_energy(a[0])
+        run = hydro.Run(nb, k["D"], k["C_min"], dt0, 2.0, steps, prec)
+        run.set_timing(True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        run.advance(a, b, steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        kms, n = run.kernel_time()
+        st = run.status()
+        run.close()
+        if st["rc"] != 0 or st["it"] != steps:
+            raise RuntimeError(f"c3 {tag} run failed: {st}")
+        cur = a if st["current"] == 0 else b
+        del a, b  # the other pair is freed; `cur` keeps the final state
+        out[tag] = {"state": cur, "t": st["t"], "gcups": cells * steps / wall / 1e9, "kernel_ms": kms / n,
+                    "mass_rel_change": abs(hydro.total_mass(cur[0]) - m0) / m0,
+                    "energy_rel_change": abs(hydro.total_energy(cur[0]) - e0) / e0}
+        log(f"[bench] c3 {tag}: {steps} steps, t={st['t']:.5e}, {out[tag]['gcups']:.2f} GCUPS, kernel "
+            f"{out[tag]['kernel_ms']:.3f} ms")
+    ref = _prims3(torch, *out["fp64"]["state"])
+    got = _prims3(torch, *out["fp32flux"]["state"])
+    vel = torch.sqrt(ref["u"] ** 2 + ref["v"] ** 2 + ref["w"] ** 2)
+    vscale = float(torch.max(torch.sqrt(1.4 * ref["p"].abs() / ref["rho"]) + vel))
+    del vel
+    errors, own = {}, {}
+    for key in ("rho", "p", "u", "v", "w"):
+        d = (got[key] - ref[key]).abs()
+        scale = vscale if key in ("u", "v", "w") else float(ref[key].abs().max())
+        errors[key] = {"linf": float(f"{float(d.max()) / scale:.3e}"),
+                       "l2": float(f"{float(torch.sqrt(torch.mean(d * d))) / scale:.3e}")}
+        if key in ("u", "v", "w"):
+            m = float(ref[key].abs().max())
+            own[key] = float(f"{float(d.max()) / m:.3e}") if m >= 1e-2 * vscale else None
+        del d
+    f = out["fp32flux"]
+    achieved = 80 * cells / (f["kernel_ms"] / 1e3) / 1e9
+    log("[bench] c3 FP32-flux vs FP64 after 200 steps, relative L-inf / L2: "
+        + ", ".join(f"{key} {e['linf']:.2e}/{e['l2']:.2e}" for key, e in errors.items()))
+    return {
+        "metric": "cell-updates/s (GCUPS) + % HBM roofline",
+        "value": round(f["gcups"], 4),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": round(1e3 * cells / (f["gcups"] * 1e9), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state, f32 flux",
+        "data": "synthetic (parameterised blast IC generated on the device)",
+        "config": {"workload": "blast3d_512_c3", "grid": list(nb), "cells": cells, "steps": steps,
+                   "ic": "sphere radius 0.1 L, p_in 10, p_out 0.1, rho 1.0 / 1.2",
+                   "boundaries": "periodic X, reflective Y/Z", "kernel": "fvm3d (FP32-flux and FP64)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic("blast3d_512_fp32flux")},
+        "fp64": {"gcups": round(out["fp64"]["gcups"], 4), "kernel_ms": round(out["fp64"]["kernel_ms"], 4),
+                 "t": out["fp64"]["t"], "mass_rel_change": out["fp64"]["mass_rel_change"],
+                 "energy_rel_change": out["fp64"]["energy_rel_change"]},
+        "fp32flux": {"gcups": round(f["gcups"], 4), "kernel_ms": round(f["kernel_ms"], 4), "t": f["t"],
+                     "mass_rel_change": f["mass_rel_change"], "energy_rel_change": f["energy_rel_change"]},
+        "error_vs_fp64": {**errors, "velocity_own_max_linf": own, "bound": 1e-5,
+                          "metric": "relative to each field's max for rho and p, to the FP64 solve's max signal "
+                                    "speed for u, v, w (and, where at least 1% of it, to each component's own max)"},
+    }
+
+
 def bench_csv(args) -> dict:
     """The CSV output path of the command-line driver at scale: the same run with and
     without output; the difference is the cost of writing one final file of nb_x * nb_y
@@ -633,6 +731,14 @@ def main() -> int:
             log("no GPU visible")
             return 1
         print(json.dumps(bench_served(args)), flush=True)
+        return 0
+    if name == "c3":
+        import torch
+
+        if not torch.cuda.is_available():
+            log("no GPU visible")
+            return 1
+        print(json.dumps(bench_c3(args)), flush=True)
         return 0
     if name in BATCHED:
         import torch
