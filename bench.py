@@ -946,11 +946,14 @@ def main() -> int:
     }
     run.close()
     if not args.no_forms and name == "blast2d_40000x20000_fp64":
-        # The batched and served forms, summarised in the same line (each has its own
-        # workload for a full line: --workload batched / served).
+        # The batched form, the 512^3 FP64 / FP32-flux comparison and the served form,
+        # summarised in the same line (each has its own workload for a full line:
+        # --workload batched / c3 / served).
         a = b = cur = src = dst = None  # noqa: F841  (free the 51 GB state first)
         torch.cuda.empty_cache()
         bt = bench_batched(args, "batched2d_256_fp32flux")
+        torch.cuda.empty_cache()
+        c3 = bench_c3(args)
         torch.cuda.empty_cache()
         sv = served_in_child(args)
         line["forms"] = {
@@ -958,6 +961,9 @@ def main() -> int:
                 "value", "unit", "ms_per_step", "plain_resident_gcups", "plain_sync_launcher_gcups",
                 "speedup_vs_plain_resident", "speedup_vs_plain_sync_launcher", "plain_sample", "error_vs_fp64")}
             | {"tiling": bt["config"]["tiling"], "roofline_achieved_gbs": bt["roofline"]["achieved"]},
+            "blast3d_512_c3": {k: c3[k] for k in ("value", "unit", "ms_per_step", "fp64", "fp32flux", "error_vs_fp64")}
+            | {"roofline_achieved_gbs": c3["roofline"]["achieved"], "roofline_frac": c3["roofline"]["frac"],
+               "note": "FP32-flux GCUPS as value; 200 steps of each form from one IC"},
             "served_mixed": {"requests": sv["config"]["requests"], "arrival_rate_per_s": sv["config"]["arrival_rate_per_s"],
                              "load": sv["config"]["load"], "served": sv["served"],
                              "sequential_plain": sv["sequential_plain"],
