@@ -842,7 +842,7 @@ def main() -> int:
     # Achievable bandwidth: the library's stream copy over the same number of bytes one
     # step reads (and writes), on buffers of the state's size.  Reuses the state's
     # second buffer pair as the destination after the run.
-    copy_gbs = copy_cached_gbs = copy_nt_gbs = read_gbs = write_gbs = None
+    copy_gbs = copy_cached_gbs = copy_nt_gbs = read_gbs = write_gbs = pattern_gbs = None
     if not args.no_copy_reference:
         src, dst = cur[0], (b if cur is a else a)[0]
         hydro.stream_copy(dst, src)
@@ -886,6 +886,19 @@ def main() -> int:
         write_gbs = nbytes / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
         log(f"[bench] stream {nbytes / 1e9:.1f} GB: copy {copy_cached_gbs:.0f} GB/s, non-temporal copy "
             f"{copy_nt_gbs:.0f} GB/s, read {read_gbs:.0f} GB/s, write {write_gbs:.0f} GB/s")
+        if name == "blast2d_40000x20000_fp64":
+            # The step's own memory schedule without its arithmetic, on the state's buffers:
+            # the ceiling of its access pattern (the same loads and stores, 64 B per cell).
+            other = b if cur is a else a
+            hydro.stream_pattern2d(cur, other)
+            torch.cuda.synchronize()
+            c0.record()
+            for _ in range(3):
+                hydro.stream_pattern2d(cur, other)
+            c1.record()
+            torch.cuda.synchronize()
+            pattern_gbs = bpc * cells / (c0.elapsed_time(c1) / 3 / 1e3) / 1e9
+            log(f"[bench] the step's access pattern without arithmetic: {pattern_gbs:.0f} GB/s")
 
     if args.calibrate_copy:
         # A plain copy of exactly 1 GiB (read 1 GiB, write 1 GiB): what the FETCH_SIZE /
@@ -935,6 +948,11 @@ def main() -> int:
         "achievable_read_gbs": round(read_gbs, 1) if read_gbs else None,
         "achievable_write_gbs": round(write_gbs, 1) if write_gbs else None,
         "frac_of_achievable": round(achieved / copy_gbs, 4) if copy_gbs else None,
+        "achievable_pattern_gbs": round(pattern_gbs, 1) if pattern_gbs else None,
+        "achievable_pattern_basis": "the step's loads and stores without its arithmetic (hydro_stream_pattern2d: same "
+                                    "workgroups, strips, block order, halo rows and rows in flight), 64 B per cell"
+                                    if pattern_gbs else None,
+        "frac_of_pattern": round(achieved / pattern_gbs, 4) if pattern_gbs else None,
         "kernel_ms": round(avg_kernel_s * 1e3, 4),
         "energy": ({"avg_watts": round(bp[0], 1), "joules_per_step": round(bp[0] * elapsed / K, 4),
                     "nj_per_cell_update": round(bp[0] * elapsed / K / cells * 1e9, 4), "window_s": round(bp[1], 3),

@@ -392,6 +392,17 @@ int hydro_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
  * counterpart in the reference (measurement only). */
 int hydro_stream_copy_nt(void* dst, const void* src, size_t bytes, void* stream);
 
+/* The 2D FP64 step's memory schedule without its arithmetic: the same workgroups,
+ * 62-column windows, strips (the "rows2d" and "occ2d" knobs and their defaults), block
+ * order and rows of loads in flight as the step kernel, loading every row the step loads
+ * (halo rows included, the X ghost planes read directly) and storing every owned cell:
+ * each interior cell of (d_rhoE, d_uv) is copied unchanged to (d_rhoE_next, d_uv_next);
+ * ghost cells of the destination are not written.  Its rate is the ceiling of the step's
+ * access pattern.  Layout and sizes as `launcher`.  No counterpart in the reference
+ * (measurement only). */
+int hydro_stream_pattern2d(const double* d_rhoE, const double* d_uv, double* d_rhoE_next, double* d_uv_next,
+                           size_t NB_X, size_t NB_Y, void* stream);
+
 /* Read-only and write-only streams over `bytes` (a multiple of 16): the achievable
  * bandwidth of each direction alone.  hydro_stream_read XOR-folds every 64-bit word of
  * src into the 256 slots of d_xor256 (device memory the caller zeroes first; the XOR of

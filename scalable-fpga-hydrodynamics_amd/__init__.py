@@ -93,6 +93,7 @@ def _declare(L: C.CDLL) -> None:
     L.hydro_stream_states.argtypes = [vp, vp]
     L.hydro_stream_read.argtypes = [vp, sz, vp, vp]
     L.hydro_stream_fill.argtypes = [vp, sz, C.c_ulonglong, vp]
+    L.hydro_stream_pattern2d.argtypes = [vp, vp, vp, vp, sz, sz, vp]
     L.hydro_batch_create.argtypes = [i32, i32, sz]
     L.hydro_batch_create.restype = vp
     L.hydro_batch_destroy.argtypes = [vp]
@@ -695,6 +696,20 @@ def stream_fill(dst, pattern: int, stream=None) -> None:
     """Write-only stream: the 64-bit `pattern` to every word of dst."""
     nbytes = dst.numel() * dst.element_size()
     _check(lib().hydro_stream_fill(_p(dst), nbytes, int(pattern) & (2**64 - 1), _stream(stream)), "hydro_stream_fill")
+
+
+def stream_pattern2d(a, b, stream=None) -> None:
+    """The 2D FP64 step's memory schedule without its arithmetic (include/hydro_hip.h,
+    hydro_stream_pattern2d): copies the interior cells of the pair `a` = (rhoE, uv) into
+    the pair `b` with the step's loads and stores; b's ghost cells are not written."""
+    import torch
+
+    _, nb, _ = _state(a[0], a[1], "stream_pattern2d a", dim=2, dtype=torch.float64)
+    _state(b[0], b[1], "stream_pattern2d b", dim=2, dtype=torch.float64)
+    _like(a[0], "stream_pattern2d b rhoE", b[0])
+    _like(a[1], "stream_pattern2d b uv", b[1])
+    _check(lib().hydro_stream_pattern2d(_p(a[0]), _p(a[1]), _p(b[0]), _p(b[1]), nb[0], nb[1], _stream(stream)),
+           "hydro_stream_pattern2d")
 
 
 def stream_release(stream=None) -> None:

@@ -97,6 +97,67 @@ __global__ void __launch_bounds__(kBlock, (kMinBlocks2<TS, TF>))
     block_max_to_slot(red, key);
 }
 
+// The FP64 step's memory schedule without its arithmetic (a measurement ceiling): the
+// same workgroups, windows, strips, block order and occupancy as fvm2d_kernel, the same
+// loads (rows i0-1 .. i1+1, kPF rows ahead of the row being stored) and the same stores
+// (the owned cells of every row); each owned cell of (rhoE, mom) is copied unchanged to
+// (rhoE_next, mom_next).  The halo rows' loads are kept alive by a select on a bit
+// pattern no state holds, so they cost what the step's do.
+__global__ void __launch_bounds__(kBlock, HYDRO_2D_MIN_BLOCKS_FP64)
+    pattern2d_kernel(const double2* __restrict__ R2, const double2* __restrict__ M2, double2* __restrict__ RN2,
+                     double2* __restrict__ MN2, Geom2 g)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const unsigned lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int wb = static_cast<int>(lb % static_cast<unsigned>(g.nwb));
+    const int strip = static_cast<int>(lb / static_cast<unsigned>(g.nwb));
+    const int win = wb * kWavesPerBlock + wv;
+    if (win >= g.nwin) return;
+    const int j = win * kOwned + lane;
+    const int jl = j <= g.nb_y + 1 ? j : g.nb_y + 1;
+    const bool owned = lane >= 1 && lane <= kOwned && j <= g.nb_y;
+    const int i0 = 1 + strip * g.rows;
+    const int i1 = min(i0 + g.rows - 1, g.nb_x);
+    const long long ys = g.ys;
+    auto clamp = [&](int r) { return min(r, i1 + 1); };
+    constexpr unsigned long long kNever = 0x7ff4dead0badf00dull;  // a NaN payload no state holds
+    double2 qr[kPF + 1], qm[kPF + 1];
+    const double2 hr = R2[static_cast<long long>(i0 - 1) * ys + jl];  // the halo row behind the strip
+    double2 rc = R2[static_cast<long long>(i0) * ys + jl], mc = M2[static_cast<long long>(i0) * ys + jl];
+    const double2 hm = M2[static_cast<long long>(i0 - 1) * ys + jl];
+#pragma unroll
+    for (int q = 0; q < kPF; ++q) {
+        const long long n = static_cast<long long>(clamp(i0 + 1 + q)) * ys + jl;
+        qr[q] = R2[n];
+        qm[q] = M2[n];
+    }
+    const bool never = __double_as_longlong(hr.x) == static_cast<long long>(kNever) ||
+                       __double_as_longlong(hm.x) == static_cast<long long>(kNever);
+    for (int i = i0; i <= i1; ++i) {
+        const long long nn = static_cast<long long>(clamp(i + 1 + kPF)) * ys + jl;
+        qr[kPF] = R2[nn];
+        qm[kPF] = M2[nn];
+        if (owned) {
+            const long long n = static_cast<long long>(i) * ys + j;
+            double2 o, p;  // component-wise selects (a select of whole vectors goes through scratch)
+            o.x = never ? hr.x : rc.x;
+            o.y = never ? hr.y : rc.y;
+            p.x = never ? hm.x : mc.x;
+            p.y = never ? hm.y : mc.y;
+            RN2[n] = o;
+            MN2[n] = p;
+        }
+        rc = qr[0];
+        mc = qm[0];
+#pragma unroll
+        for (int q = 0; q < kPF; ++q) {
+            qr[q] = qr[q + 1];
+            qm[q] = qm[q + 1];
+        }
+    }
+}
+
 // Ghost fill: X planes (periodic copy or reflective mirror), hydro2d/main.cxx:349-378.
 template <typename T>
 __global__ void ghost_x2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long long nb_x, long long ys, int bc_x,
@@ -394,6 +455,43 @@ extern "C" int hydro_step2d(const double* d_rhoE, const double* d_uv, double* d_
     const Consts<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };
     return launch_step2d<double>(d_rhoE, d_uv, d_rhoE_next, d_uv_next, c, NB_X, NB_Y, DtDx, DtDy, bc_y, red, nullptr,
                                  Dt_next, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int hydro_stream_pattern2d(const double* d_rhoE, const double* d_uv, double* d_rhoE_next, double* d_uv_next,
+                                      size_t NB_X, size_t NB_Y, void* stream)
+{
+    clear_error();
+    if (!d_rhoE || !d_uv || !d_rhoE_next || !d_uv_next) {
+        set_error("hydro_stream_pattern2d: null pointer argument");
+        return HYDRO_ERR_ARG;
+    }
+    if (NB_X < 1 || NB_Y < 1 || NB_X > (1u << 30) || NB_Y > (1u << 30)) {
+        set_error("hydro_stream_pattern2d: interior sizes must be in [1, 2^30], got %zu x %zu", NB_X, NB_Y);
+        return HYDRO_ERR_ARG;
+    }
+    // The FP64 step's geometry (launch_step2d): windows, strip rows, workgroups per CU.
+    Geom2 g{};
+    g.nb_x = static_cast<int>(NB_X);
+    g.nb_y = static_cast<int>(NB_Y);
+    g.ys = static_cast<long long>(NB_Y) + 2;
+    g.nwin = static_cast<int>((NB_Y + kOwned - 1) / kOwned);
+    g.nwb = (g.nwin + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int rows_set = g_rows_2d.load(std::memory_order_relaxed);
+    const int occ_set = g_occ_2d.load(std::memory_order_relaxed);
+    const int occ = occ_set > 0 ? occ_set : default_occ_2d<double, double>();
+    g.rows = rows_set > 0 ? rows_set : default_rows_2d<double, double>();
+    while (g.rows > 8 && (static_cast<long long>(NB_X) + g.rows - 1) / g.rows * g.nwb < kFillBlocks2) g.rows /= 2;
+    const long long nblocks = (static_cast<long long>(NB_X) + g.rows - 1) / g.rows * g.nwb;
+    if (nblocks > 0x7fffffffLL) {
+        set_error("hydro_stream_pattern2d: grid too large (%lld workgroups)", nblocks);
+        return HYDRO_ERR_ARG;
+    }
+    hipLaunchKernelGGL(pattern2d_kernel, dim3(static_cast<unsigned>(nblocks)), dim3(kBlock),
+                       occupancy_lds<pattern2d_kernel>(occ), static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const double2*>(d_rhoE), reinterpret_cast<const double2*>(d_uv),
+                       reinterpret_cast<double2*>(d_rhoE_next), reinterpret_cast<double2*>(d_uv_next), g);
+    HYDRO_HIP_OR(hipGetLastError(), HYDRO_ERR_HIP);
+    return 0;
 }
 
 extern "C" int hydro_ghost_fill2d(double* d_rhoE, double* d_uv, size_t NB_X, size_t NB_Y, int bc_x, int bc_y,

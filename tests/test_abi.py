@@ -70,6 +70,8 @@ def test_bad_arguments_are_rejected_without_a_device(hydro):
     with pytest.raises(hydro.HydroError):
         hydro.tune_set("rows2d", 1)
     assert L.hydro_run_advance(None, None, None, None, None, 1, None) == -1
+    assert L.hydro_stream_pattern2d(None, None, None, None, 10, 10, None) == -1
+    assert "null pointer" in hydro.last_error()
     assert L.hydro_run_trajectory(None, 0, 1, None, None, None) == -1
 
 

@@ -46,3 +46,20 @@ def test_stream_fill_exact(gpu, n16):
     torch.cuda.synchronize()
     assert torch.equal(dst[: 2 * n16], torch.full((2 * n16,), pattern, dtype=torch.int64, device="cuda"))
     assert torch.equal(dst[2 * n16:], torch.full((2,), -1, dtype=torch.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("nb", [(1, 1), (5, 70), (37, 125), (997, 613), (4000, 2000)])
+def test_stream_pattern2d_copies_the_interior(gpu, nb):
+    """The step-pattern copy moves every interior cell of both arrays and writes no ghost
+    cell: grids of one cell, ragged windows and strips, and the 4000x2000 comparison grid."""
+    shape = (nb[0] + 2, nb[1] + 2, 2)
+    g = torch.Generator(device="cuda").manual_seed(nb[0] * 7919 + nb[1])
+    a = [torch.rand(shape, dtype=torch.float64, device="cuda", generator=g) for _ in range(2)]
+    b = [torch.full(shape, -7.0, dtype=torch.float64, device="cuda") for _ in range(2)]
+    gpu.stream_pattern2d(a, b)
+    torch.cuda.synchronize()
+    for src, dst in zip(a, b):
+        assert torch.equal(dst[1:-1, 1:-1], src[1:-1, 1:-1])
+        ghost = torch.ones(shape[:2], dtype=torch.bool, device="cuda")
+        ghost[1:-1, 1:-1] = False
+        assert bool((dst[ghost] == -7.0).all())
