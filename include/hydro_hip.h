@@ -194,8 +194,8 @@ int hydro_run_noalloc(int dim, const size_t* nb, const double* D, double C_min, 
  *   "occ2d"  = workgroups per CU of the 2D kernel (3..8, enforced by reserving LDS;
  *              8 leaves the count to the kernel's registers);
  *   "rows3d" = maximum X rows per workgroup strip of the 3D kernel (2..4096).
- * 0 selects the per-precision defaults: 2D 12 rows with 4 (FP64) or 3 (FP32-flux)
- * workgroups per CU, 16 rows with 5 for FP32; 3D 32 rows, 16 for FP32-flux.  Grids with few workgroups
+ * 0 selects the per-precision defaults: 2D 12 rows with 3 workgroups per CU (FP64 and
+ * FP32-flux), 16 rows with 5 for FP32; 3D 32 rows.  Grids with few workgroups
  * use shorter strips than the maximum.  hydro_tune_get returns a knob's setting, or with
  * the suffix "_last" ("rows2d_last", "occ2d_last", "rows3d_last") the geometry of the
  * last step launched.  Returns 0, or HYDRO_ERR_ARG for an unknown key or out-of-range

@@ -212,11 +212,12 @@ __global__ void ghost_y2d_kernel(T* __restrict__ rhoE, T* __restrict__ uv, long 
 std::atomic<int> g_rows_2d{0}, g_occ_2d{0};
 std::atomic<int> g_last_rows_2d{0}, g_last_occ_2d{0};
 
-// Defaults from interleaved sweeps of the 40000x20000 blast on MI355X (strips of 8..64
-// rows x 3..8 workgroups per CU, three boxes): short strips and fewer waves in flight
-// keep fewer DRAM rows open at once.  12-row strips with 4 (FP64) or 3 (FP32-flux)
-// workgroups per CU and 16-row strips with 5 (FP32) run 7-19% faster than 64-row
-// strips at full occupancy.
+// Defaults from sweeps of the 40000x20000 blast on MI355X (strips of 8..64 rows x 3..8
+// workgroups per CU, the configurations alternated inside one process): short strips and
+// fewer waves in flight keep fewer DRAM rows open at once.  With one row of loads ahead
+// (HYDRO_2D_PREFETCH), 12-row strips with 3 workgroups per CU (FP64, FP32-flux) and
+// 16-row strips with 5 (FP32 state) are the fastest: FP64 9.55 ms against 9.79 at 4 per
+// CU and 10.15 at 5; 64-row strips at full occupancy ran 7-19% slower.
 template <typename T, typename TF>
 constexpr int default_rows_2d()
 {
@@ -226,8 +227,7 @@ template <typename T, typename TF>
 constexpr int default_occ_2d()
 {
     if constexpr (std::is_same_v<T, float>) return 5;
-    if constexpr (std::is_same_v<TF, float>) return 3;
-    return 4;
+    return 3;
 }
 
 // Workgroups a step should have to fill the chip (8 per CU on 256 CUs); smaller grids

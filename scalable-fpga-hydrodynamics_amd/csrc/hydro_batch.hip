@@ -73,7 +73,10 @@ namespace {
 // Rows of loads in flight in the batched march: 2, one fewer than the plain step's.
 // Its tiles are 16 rows long; on the 256-blast mix 1 or 2 rows ran 0.5% faster than 3,
 // and 4 or 5 slower.
-constexpr int kPFBatch = 2;
+#ifndef HYDRO_BATCH_PREFETCH
+#define HYDRO_BATCH_PREFETCH 2
+#endif
+constexpr int kPFBatch = HYDRO_BATCH_PREFETCH;
 template <typename TF>
 constexpr int kBatchWaves2 = 4;
 

@@ -23,17 +23,22 @@ namespace hydro {
 
 constexpr int kOwned = kWave - 2;  // owned columns per wave window
 constexpr int kWavesPerBlock = 4;
-// Workgroups per CU the 2D step must fit in registers: 5 (at most 102 VGPRs), and 4 for
-// the FP64 scheme (102 VGPRs; at 5 it spills 6 registers, 7% slower).
+// Workgroups per CU the 2D step must fit in registers (the launch bound; the launch
+// itself runs fewer, see default_occ_2d): 5 (at most 102 VGPRs), and 4 for the FP64 scheme
+// (at most 128; it uses 90 with one row of loads ahead).
 #ifndef HYDRO_2D_MIN_BLOCKS
 #define HYDRO_2D_MIN_BLOCKS 5
 #endif
 #ifndef HYDRO_2D_MIN_BLOCKS_FP64
 #define HYDRO_2D_MIN_BLOCKS_FP64 4
 #endif
-// Rows in flight ahead of the row being converted (register prefetch ring depth).
+// Rows of loads issued ahead of the row being converted (register prefetch ring depth).
+// The ring advances by register moves, before which the compiler waits for the loads in
+// flight, so about one row is in flight whatever the depth; depth 1 needs the fewest
+// registers and moves (FP64 90 VGPRs) and ran 0.3% (FP64), 2.2% (FP32-flux) and 0.7% (FP32
+// state) faster than 3 on 40000x20000, alternated in one process.
 #ifndef HYDRO_2D_PREFETCH
-#define HYDRO_2D_PREFETCH 3
+#define HYDRO_2D_PREFETCH 1
 #endif
 constexpr int kPF = HYDRO_2D_PREFETCH;
 // Non-temporal (streaming) stores of the new state.
