@@ -22,7 +22,13 @@
 namespace hydro {
 
 constexpr int kOwned = kWave - 2;  // owned columns per wave window
-constexpr int kWavesPerBlock = 4;
+// Waves (adjacent column windows) per workgroup: 4, a 4 KiB row segment per array.
+// Workgroups of 2 waves at the same waves per CU ran the FP64 step 6% slower, of 8
+// waves 3% slower (40000x20000, alternated in one process).
+#ifndef HYDRO_2D_WAVES
+#define HYDRO_2D_WAVES 4
+#endif
+constexpr int kWavesPerBlock = HYDRO_2D_WAVES;
 // Workgroups per CU the 2D step must fit in registers (the launch bound; the launch
 // itself runs fewer, see default_occ_2d): 5 (at most 102 VGPRs), and 4 for the FP64 scheme
 // (at most 128; it uses 90 with one row of loads ahead).
