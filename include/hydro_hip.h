@@ -195,7 +195,7 @@ int hydro_run_noalloc(int dim, const size_t* nb, const double* D, double C_min, 
  *              8 leaves the count to the kernel's registers);
  *   "rows3d" = maximum X rows per workgroup strip of the 3D kernel (2..4096).
  * 0 selects the per-precision defaults: 2D 12 rows with 3 workgroups per CU (FP64 and
- * FP32-flux), 16 rows with 5 for FP32; 3D 32 rows.  Grids with few workgroups
+ * FP32-flux), 16 rows with 6 for FP32; 3D 32 rows.  Grids with few workgroups
  * use shorter strips than the maximum.  hydro_tune_get returns a knob's setting, or with
  * the suffix "_last" ("rows2d_last", "occ2d_last", "rows3d_last") the geometry of the
  * last step launched.  Returns 0, or HYDRO_ERR_ARG for an unknown key or out-of-range

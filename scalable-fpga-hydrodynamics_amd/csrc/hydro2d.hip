@@ -50,6 +50,11 @@ struct Geom2 {
 // fluxes and signal speeds.  <double, double, IeeeMath> is the reference FP64 scheme,
 // <float, float, IeeeMath> its USE_FLOAT build, <double, float, FastMath> the FP32-flux
 // form (FP32 fluxes, increments and update accumulated in FP64).
+// Rows of loads ahead (HYDRO_2D_PREFETCH) for the FP64 state; the FP32 state's loads are
+// issued where they are used (5.06 vs 5.11 ms on 40000x20000, alternated in one process).
+template <typename TS>
+constexpr int kPF2 = std::is_same_v<TS, float> ? 0 : kPF;
+
 template <typename TS, typename TF>
 constexpr int kMinBlocks2 =
     std::is_same_v<TS, double> && std::is_same_v<TF, double> ? HYDRO_2D_MIN_BLOCKS_FP64 : HYDRO_2D_MIN_BLOCKS;
@@ -91,8 +96,8 @@ __global__ void __launch_bounds__(kBlock, (kMinBlocks2<TS, TF>))
 
     uint64_t key = 0;
     if (win < g.nwin)
-        key = march2d<TS, TF, M, false>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned, g.wrap_x != 0,
-                                        gamma, gmo, divgamma, K, DtDx, DtDy, key);
+        key = march2d<TS, TF, M, false, kPF2<TS>>(R2, M2, RN2, MN2, g.ys, g.nb_x, i0, i1, j, jl, ymask, owned,
+                                                  g.wrap_x != 0, gamma, gmo, divgamma, K, DtDx, DtDy, key);
 
     block_max_to_slot(red, key);
 }
@@ -215,9 +220,10 @@ std::atomic<int> g_last_rows_2d{0}, g_last_occ_2d{0};
 // Defaults from sweeps of the 40000x20000 blast on MI355X (strips of 8..64 rows x 3..8
 // workgroups per CU, the configurations alternated inside one process): short strips and
 // fewer waves in flight keep fewer DRAM rows open at once.  With one row of loads ahead
-// (HYDRO_2D_PREFETCH), 12-row strips with 3 workgroups per CU (FP64, FP32-flux) and
-// 16-row strips with 5 (FP32 state) are the fastest: FP64 9.55 ms against 9.79 at 4 per
-// CU and 10.15 at 5; 64-row strips at full occupancy ran 7-19% slower.
+// (HYDRO_2D_PREFETCH), 12-row strips with 3 workgroups per CU (FP64, FP32-flux) and, with
+// no row ahead, 16-row strips with 6 (FP32 state) are the fastest: FP64 9.55 ms against
+// 9.79 at 4 per CU and 10.15 at 5; FP32 state 5.04 ms against 5.06 at 5 and 5.12 at 7;
+// 64-row strips at full occupancy ran 7-19% slower.
 template <typename T, typename TF>
 constexpr int default_rows_2d()
 {
@@ -226,7 +232,7 @@ constexpr int default_rows_2d()
 template <typename T, typename TF>
 constexpr int default_occ_2d()
 {
-    if constexpr (std::is_same_v<T, float>) return 5;
+    if constexpr (std::is_same_v<T, float>) return 6;
     return 3;
 }
 
