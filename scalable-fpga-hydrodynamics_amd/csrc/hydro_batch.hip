@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kBlock3, 4)
         const int kl = k < 1 ? 1 : (k > nb_z ? nb_z : k);
         const unsigned int zmask = (k == 0 || k == nb_z + 1) ? kSignMask<TF> : 0u;
         const long long col = static_cast<long long>(jl) * B->zs + kl;
-        key = march3d<TS, TF, M>(static_cast<const V2*>(B->rhoE[cur]), static_cast<const Trip<TS>*>(B->mom[cur]),
+        key = march3d<TS, TF, M, kPF3>(static_cast<const V2*>(B->rhoE[cur]), static_cast<const Trip<TS>*>(B->mom[cur]),
                                  static_cast<V2*>(B->rhoE[cur ^ 1u]), static_cast<Trip<TS>*>(B->mom[cur ^ 1u]), B->zy,
                                  col, nb_x, sg.i0, sg.i1, w, lane, halo_wave, owned, ymask, zmask, true, gamma, gmo,
                                  divgamma, K, DtDx, DtDy, DtDz, lds, key);

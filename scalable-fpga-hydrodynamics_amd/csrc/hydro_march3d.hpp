@@ -56,6 +56,11 @@ __host__ __device__ __forceinline__ int z_window_width(int nb_z)
 #ifndef HYDRO_3D_MIN_WAVES_FP32
 #define HYDRO_3D_MIN_WAVES_FP32 8
 #endif
+// Minimum waves per SIMD of the plain FP32-flux step: 3 (one 12-wave workgroup per CU).
+// At 6 (two per CU, within 80 VGPRs with no load ahead) it ran 5% slower on 512^3.
+#ifndef HYDRO_3D_MIN_WAVES_FLUX
+#define HYDRO_3D_MIN_WAVES_FLUX 3
+#endif
 constexpr int kTY = HYDRO_3D_TY;           // owned Y rows per workgroup (FP64, FP32 state, batched)
 constexpr int kTYFlux = HYDRO_3D_TY_FLUX;  // owned Y rows per workgroup of the plain FP32-flux step
 constexpr int kPF3 = HYDRO_3D_PREFETCH;
@@ -172,8 +177,9 @@ struct March3Lds {
 };
 
 // March rows i0..i1 of one (Y x Z) tile; see the file comment.  `col` is the in-plane
-// offset jl * zs + kl of the column this lane loads; stores go to the same column.
-template <typename TS, typename TF, typename M, int NW>
+// offset jl * zs + kl of the column this lane loads; stores go to the same column.  PF
+// rows of loads run ahead of row i+1 (0: each row's loads are issued where it is used).
+template <typename TS, typename TF, typename M, int PF, int NW>
 __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __restrict__ R2,
                                             const Trip<TS>* __restrict__ M3,
                                             typename Vec2_3<TS>::type* __restrict__ RN2,
@@ -185,15 +191,15 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
     using V2 = typename Vec2_3<TS>::type;
     auto& s_prim = lds.prim;
     auto& s_flux = lds.flux;
-    // Row i in rc/mc; rows i+1 .. i+kPF3 in the ring qr/qm (slot kPF3 takes the next
+    // Row i in rc/mc; rows i+1 .. i+PF in the ring qr/qm (slot PF takes the next
     // load each step).
-    V2 rc, qr[kPF3 + 1];
-    Trip<TS> mc, qm[kPF3 + 1];
+    V2 rc, qr[PF + 1];
+    Trip<TS> mc, qm[PF + 1];
     P6<TF> Pc;
     TF fx1_0 = TF(0), fx1_1 = TF(0), fx1_2 = TF(0), fx1_3 = TF(0), fx1_4 = TF(0);
     load3<TS>(R2, M3, row3_of(i0, nb_x, wrap_x) * zy + col, rc, mc);
 #pragma unroll
-    for (int q = 0; q < kPF3; ++q)
+    for (int q = 0; q < PF; ++q)
         load3<TS>(R2, M3, row3_of(min(i0 + 1 + q, i1 + 1), nb_x, wrap_x) * zy + col, qr[q], qm[q]);
     to_prim<TS, TF, M>(rc, mc, gmo, gamma, Pc);
     if (!halo_wave) {
@@ -206,8 +212,8 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
     }
 
     for (int i = i0; i <= i1; ++i) {
-        const int ipf = min(i + 1 + kPF3, i1 + 1);
-        load3<TS>(R2, M3, row3_of(ipf, nb_x, wrap_x) * zy + col, qr[kPF3], qm[kPF3]);
+        const int ipf = min(i + 1 + PF, i1 + 1);
+        load3<TS>(R2, M3, row3_of(ipf, nb_x, wrap_x) * zy + col, qr[PF], qm[PF]);
         const V2 rn = qr[0];
         const Trip<TS> mn = qm[0];
 
@@ -333,7 +339,7 @@ __device__ __forceinline__ uint64_t march3d(const typename Vec2_3<TS>::type* __r
         rc = rn;
         mc = mn;
 #pragma unroll
-        for (int q = 0; q < kPF3; ++q) {
+        for (int q = 0; q < PF; ++q) {
             qr[q] = qr[q + 1];
             qm[q] = qm[q + 1];
         }
