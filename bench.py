@@ -533,12 +533,11 @@ def bench_served(args) -> dict:
 
 def served_in_child(args) -> dict:
     """The served measurement in a fresh process (python bench.py --workload served), as
-    its own workload runs it.  Run in the same process after the batched FP32-flux form
-    (its error check against an FP64 batch and its one-at-a-time FP32-flux solves), the
-    served p99 came out 0.64 s instead of 0.48 s in four runs, while the sequential
-    baseline measured in that process did not move; after the FP64 batched form, or the
-    FP32-flux batch alone, it did not.  The cause is not found; a fresh process measures
-    the served form by itself."""
+    its own workload runs it, so that it shares nothing with the forms measured before
+    it.  (Run after the batched form, the served p99 once came out 0.64 s instead of
+    0.48 s: the streams created before had changed which of HIP's 4 hardware queues the
+    server's streams shared, and an engine's ticks then waited behind a large solve
+    enqueued whole.  Large solves are now fed in chunks, which removed that.)"""
     import subprocess
 
     cmd = [sys.executable, str(pathlib.Path(__file__).resolve()), "--workload", "served", "--requests",
