@@ -103,6 +103,51 @@ def test_mixed_requests_same_alone_and_in_the_mix(gpu, oracle, large_s, backlog)
     s.close()
 
 
+def test_large_solves_fed_in_chunks_match_the_oracle(gpu, oracle):
+    """The server enqueues a large solve about 2 ms of estimated work at a time (at most
+    two chunks in flight).  Large requests that span several chunks, beside small batched
+    ones, give the oracle's fields, t and iteration count bit for bit, including one that
+    stops on max_t inside a chunk (the remaining steps of its chunks do nothing)."""
+    C2, C3 = 0.5, 0.4
+    # 1000 x 800: about 19 us per step by the estimate, so 104-step chunks; stop at 170.
+    r_stop, m_stop, dt0 = oracle.ic2d(1000, 800, C_cfl=C2, ic_mode=1, radius_frac=0.1, p_in=10.0)
+    stop = oracle.run2d(r_stop, m_stop, dt0, 1e30, 170, C_cfl=C2, nthreads=8)
+    large = [((1200, 900), 400, 1e30), ((96, 96, 96), 200, 1e30), ((1000, 800), 500, stop["t"])]
+    small = [((64, 48), 50), ((40, 30, 20), 40), ((90, 70), 60), ((33, 65), 45)]
+    for nb, steps, _ in large:
+        assert _solo_estimate({"nb": nb, "steps": steps}) > 2 * 2e-3  # several chunks each
+    s = gpu.Server(1 << 32, capacity=8, steps_per_tick=9)
+    s.set_policy(1e-3)
+    ids = [s.submit(nb, gpu.FP64, C=C2 if len(nb) == 2 else C3, radius_frac=0.1, p_in=10.0, max_t=max_t,
+                    max_it=steps, keep=True) for nb, steps, max_t in large]
+    for nb, steps in small:
+        assert _solo_estimate({"nb": nb, "steps": steps}) < 1e-3
+        s.submit(nb, gpu.FP64, max_it=steps)
+        s.tick()
+    while s.pending()[:2] != (0, 0):
+        s.tick()
+    for (nb, steps, max_t), rid in zip(large, ids):
+        info = s.request(rid)
+        if max_t < 1e30:
+            ro, mo, res = r_stop, m_stop, stop
+        elif len(nb) == 2:
+            ro, mo, d0 = oracle.ic2d(*nb, C_cfl=C2, ic_mode=1, radius_frac=0.1, p_in=10.0)
+            res = oracle.run2d(ro, mo, d0, 1e30, steps, C_cfl=C2, nthreads=8)
+        else:
+            ro, mo, d0 = oracle.ic3d(*nb, C_cfl=C3, ic_mode=1, radius_frac=0.1, p_in=10.0)
+            res = oracle.run3d(ro, mo, d0, 1e30, steps, C_cfl=C3, nthreads=8)
+        print(f"[serve] large {nb}: it {info['it']} (oracle {res['it']}), t {info['t']!r}", flush=True)
+        assert info["rc"] == 0 and info["state"] == s.DONE, info
+        assert info["it"] == res["it"] and info["t"] == res["t"]
+        mix = _result(s, rid, nb)
+        sl = (slice(1, -1),) * len(nb)
+        np.testing.assert_array_equal(mix[0].view(np.uint64), ro[sl].view(np.uint64), err_msg=str(nb))
+        np.testing.assert_array_equal(mix[1].view(np.uint64), mo[sl].view(np.uint64), err_msg=str(nb))
+        s.release(rid)
+    assert stop["it"] == 170
+    s.close()
+
+
 def test_admission_respects_the_hbm_budget(gpu):
     reqs = workloads.served_requests(6, rate_per_s=1.0, seed=5, frac_2d=1.0, side2d=(200, 256), steps=(30, 60))
     need = max(workloads.state_bytes(r["nb"]) for r in reqs)
