@@ -354,7 +354,8 @@ int hydro_server_set_limits(hydro_server* server, double max_active_cells, size_
 /* Size classes: a request whose estimated solve time alone (max_it steps of the plain
  * resident loop, each a launch floor plus a cost per interior cell measured on MI355X)
  * is at least large_seconds is "large".  Large requests run one at a time, in arrival
- * order, as plain device-resident solves on a stream of their own; smaller requests are
+ * order, as plain device-resident solves on a stream of their own, their steps enqueued
+ * by hydro_server_tick about 2 ms of estimated work at a time; smaller requests are
  * admitted to the batched engines beside them at once.  0 (the default) puts every
  * request in one class.  Applies to requests submitted after the call. */
 int hydro_server_set_policy(hydro_server* server, double large_seconds);
