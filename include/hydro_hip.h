@@ -158,7 +158,8 @@ int hydro_run_status(hydro_run* run, size_t* it, double* t, double* dt, int* don
 int hydro_run_trajectory(hydro_run* run, size_t from, size_t count, double* t_out, double* dt_out, void* stream);
 
 /* Record HIP events around every step-kernel launch of later hydro_run_advance calls
- * (enable != 0), for measuring the kernel alone. */
+ * (enable != 0), for measuring the kernel alone.  At most 65536 launches are recorded
+ * between two hydro_run_kernel_time calls; later ones run untimed. */
 int hydro_run_set_timing(hydro_run* run, int enable);
 
 /* Wait for the recorded launches and return the sum of their kernel times in ms and

@@ -46,6 +46,8 @@ struct hydro_run {
 
 namespace {
 
+constexpr size_t kMaxTimed = 65536;  // launches timed between two hydro_run_kernel_time calls
+
 // gamma, K and the derived constants of hydro2d/main.cxx:601-606 in the state precision.
 void physics_constants(int precision, double C_min, double out[5])
 {
@@ -198,7 +200,10 @@ try {
         void* nr = in_a ? rhoE_b : rhoE_a;
         void* nm = in_a ? mom_b : mom_a;
         int rc;
-        if (r->timing) {
+        // Timed launches until the record holds kMaxTimed of them (events are reused after
+        // each hydro_run_kernel_time, so the record never grows past that).
+        const bool timed = r->timing && r->ev_used < kMaxTimed;
+        if (timed) {
             if (r->ev.size() < 2 * (r->ev_used + 1)) {
                 for (int q = 0; q < 2; ++q) {
                     hipEvent_t e = nullptr;
@@ -207,8 +212,8 @@ try {
                 }
             }
         }
-        hipEvent_t ev_a = r->timing ? r->ev[2 * r->ev_used] : nullptr;
-        hipEvent_t ev_b = r->timing ? r->ev[2 * r->ev_used + 1] : nullptr;
+        hipEvent_t ev_a = timed ? r->ev[2 * r->ev_used] : nullptr;
+        hipEvent_t ev_b = timed ? r->ev[2 * r->ev_used + 1] : nullptr;
         rc = enqueue_step(r->dim, r->nb, r->bc, r->precision, r->consts, r->d_ctl, r->d_traj_t, r->d_traj_dt, cr, cm,
                           nr, nm, s, ev_a, ev_b);
         if (!rc && ev_b) ++r->ev_used;

@@ -214,6 +214,40 @@ def test_resident_run2d_stops_on_time(gpu, oracle):
     assert_bitwise(interior2(gr), interior2(rhoE), "rhoE after timed stop")
 
 
+def test_resident_run_timing_changes_nothing_and_is_bounded(gpu):
+    """HIP events around each step kernel (hydro_run_set_timing): the timed run ends in
+    the untimed run's state bit for bit, every launch is counted, and at most 65536
+    launches are recorded between two reads (later ones run untimed)."""
+    nb = (6, 5)
+    k = gpu.constants(nb, 0.5, "f64")
+    shape = (nb[0] + 2, nb[1] + 2, 2)
+    finals = []
+    for timing in (False, True):
+        a = [torch.zeros(shape, dtype=torch.float64, device="cuda") for _ in range(2)]
+        b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+        dt0 = gpu.ic(a[0], a[1], k["C_min"])
+        run = gpu.Run(nb, k["D"], k["C_min"], dt0, 1e30, 200, gpu.FP64)
+        run.set_timing(timing)
+        run.advance(a, b, 40)
+        if timing:
+            ms, n = run.kernel_time()
+            assert n == 40 and ms > 0
+            run.advance(a, b, 65536 + 7)  # past max_it: the extra steps do nothing
+            ms, n = run.kernel_time()
+            assert n == 65536
+            run.advance(a, b, 3)
+            assert run.kernel_time()[1] == 3
+        else:
+            run.advance(a, b, 65536 + 10)
+        st = run.status()
+        assert st["it"] == 200 and st["done"]
+        cur = a if st["current"] == 0 else b
+        finals.append((st["t"], cur[0].cpu(), cur[1].cpu()))
+        run.close()
+    assert finals[0][0] == finals[1][0]
+    assert torch.equal(finals[0][1], finals[1][1]) and torch.equal(finals[0][2], finals[1][2])
+
+
 @pytest.mark.parametrize("nb,bc", [((20, 20, 20), (0, 1, 1)), ((30, 14, 40), (1, 0, 1)), ((16, 16, 16), (0, 0, 0))])
 def test_resident_run3d(gpu, oracle, nb, bc):
     rhoE, uvw, dt0 = oracle.ic3d(*nb)
