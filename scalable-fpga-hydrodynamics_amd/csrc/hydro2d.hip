@@ -240,23 +240,35 @@ constexpr int default_occ_2d()
 // use shorter strips (down to 8 rows) until they have that many.
 constexpr long long kFillBlocks2 = 2048;
 
-template <typename T, typename TF = T, typename M = IeeeMath>
-int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
-                  T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,
-                  hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr, int wraps = 0)
+// Argument checks of a 2D step, run by the plain entry points before they bind the
+// stream's reduction state (so that a rejected call leaves nothing behind) and by
+// launch_step2d.  `resident`: Dt lives in the run's control block, not in Dt_next.
+int check_step2d(const void* rhoE, const void* uv, const void* rhoE_next, const void* uv_next, bool resident,
+                 const void* Dt_next, size_t nb_x, size_t nb_y, int bc_y)
 {
-    if (!rhoE || !uv || !rhoE_next || !uv_next || (!ctl && !Dt_next)) {
+    if (!rhoE || !uv || !rhoE_next || !uv_next || (!resident && !Dt_next)) {
         set_error("hydro 2D step: null pointer argument");
         return HYDRO_ERR_ARG;
     }
-    if (nb_x < 1 || nb_y < 1 || nb_x > (1u << 30) || nb_y > (1u << 30)) {
-        set_error("hydro 2D step: interior sizes must be in [1, 2^30], got %zu x %zu", nb_x, nb_y);
+    const size_t nbv[2] = { nb_x, nb_y };
+    if (nb_x < 1 || nb_y < 1 || nb_x > (1u << 30) || nb_y > (1u << 30) || !grid_fits(2, nbv)) {
+        set_error("hydro 2D step: interior sizes must be in [1, 2^30] with at most 2^40 cells, got %zu x %zu", nb_x,
+                  nb_y);
         return HYDRO_ERR_ARG;
     }
     if (bc_y != HYDRO_BC_PERIODIC && bc_y != HYDRO_BC_REFLECTIVE) {
         set_error("hydro 2D step: unknown bc_y %d", bc_y);
         return HYDRO_ERR_ARG;
     }
+    return 0;
+}
+
+template <typename T, typename TF = T, typename M = IeeeMath>
+int launch_step2d(const T* rhoE, const T* uv, T* rhoE_next, T* uv_next, const Consts<T>& c, size_t nb_x, size_t nb_y,
+                  T DtDx, T DtDy, int bc_y, StepReduce* red, StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt,
+                  hipStream_t s, hipEvent_t ev_a = nullptr, hipEvent_t ev_b = nullptr, int wraps = 0)
+{
+    if (const int rc = check_step2d(rhoE, uv, rhoE_next, uv_next, ctl != nullptr, Dt_next, nb_x, nb_y, bc_y)) return rc;
     Geom2 g;
     g.nb_x = static_cast<int>(nb_x);
     g.nb_y = static_cast<int>(nb_y);
@@ -316,7 +328,8 @@ template <typename T>
 int launch_ghost2d(T* rhoE, T* uv, size_t nb_x, size_t nb_y, int bc_x, int bc_y, const StepControl* ctl,
                    hipStream_t s)
 {
-    if (!rhoE || !uv || nb_x < 1 || nb_y < 1) {
+    const size_t nbv[2] = { nb_x, nb_y };
+    if (!rhoE || !uv || nb_x < 1 || nb_y < 1 || !grid_fits(2, nbv)) {
         set_error("hydro ghost fill 2D: bad arguments");
         return HYDRO_ERR_ARG;
     }
@@ -362,6 +375,9 @@ double timed_launcher2d(const T* d_rhoE, const T* d_uv, T* d_rhoE_next, T* d_uv_
                         T gamma_minus_one, T divgamma, T K, size_t NB_X, size_t NB_Y, T DtDx, T DtDy, void* stream)
 {
     clear_error();
+    if (const int rc = check_step2d(d_rhoE, d_uv, d_rhoE_next, d_uv_next, false, Dt_next, NB_X, NB_Y,
+                                    HYDRO_BC_REFLECTIVE))
+        return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StepReduce* red = plain_reduce_state(s);
     EventPair* ev = timing_events();
@@ -456,6 +472,7 @@ extern "C" int hydro_step2d(const double* d_rhoE, const double* d_uv, double* d_
                             double K, size_t NB_X, size_t NB_Y, double DtDx, double DtDy, int bc_y, void* stream)
 {
     clear_error();
+    if (const int rc = check_step2d(d_rhoE, d_uv, d_rhoE_next, d_uv_next, false, Dt_next, NB_X, NB_Y, bc_y)) return rc;
     StepReduce* red = plain_reduce_state(static_cast<hipStream_t>(stream));
     if (!red) return HYDRO_ERR_HIP;
     const Consts<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };
@@ -471,8 +488,10 @@ extern "C" int hydro_stream_pattern2d(const double* d_rhoE, const double* d_uv, 
         set_error("hydro_stream_pattern2d: null pointer argument");
         return HYDRO_ERR_ARG;
     }
-    if (NB_X < 1 || NB_Y < 1 || NB_X > (1u << 30) || NB_Y > (1u << 30)) {
-        set_error("hydro_stream_pattern2d: interior sizes must be in [1, 2^30], got %zu x %zu", NB_X, NB_Y);
+    const size_t nbv[2] = { NB_X, NB_Y };
+    if (NB_X < 1 || NB_Y < 1 || NB_X > (1u << 30) || NB_Y > (1u << 30) || !grid_fits(2, nbv)) {
+        set_error("hydro_stream_pattern2d: interior sizes must be in [1, 2^30] with at most 2^40 cells, got %zu x %zu",
+                  NB_X, NB_Y);
         return HYDRO_ERR_ARG;
     }
     // The FP64 step's geometry (launch_step2d): windows, strip rows, workgroups per CU.

@@ -251,18 +251,20 @@ int resident_per_xcd(KernelFn kernel, int block)
     return per_cu > 0 && cus >= 8 ? per_cu * (cus / 8) : 0;
 }
 
-template <typename T, typename TF = T, typename M = IeeeMath>
-int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
-                  size_t nb_y, size_t nb_z, T DtDx, T DtDy, T DtDz, int bc_y, int bc_z, StepReduce* red,
-                  StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
-                  hipEvent_t ev_b = nullptr, int wraps = 0)
+// Argument checks of a 3D step, run by the plain entry points before they bind the
+// stream's reduction state and by launch_step3d (see check_step2d).
+int check_step3d(const void* rhoE, const void* uvw, const void* rhoE_next, const void* uvw_next, bool resident,
+                 const void* Dt_next, size_t nb_x, size_t nb_y, size_t nb_z, int bc_y, int bc_z)
 {
-    if (!rhoE || !uvw || !rhoE_next || !uvw_next || (!ctl && !Dt_next)) {
+    if (!rhoE || !uvw || !rhoE_next || !uvw_next || (!resident && !Dt_next)) {
         set_error("hydro 3D step: null pointer argument");
         return HYDRO_ERR_ARG;
     }
-    if (nb_x < 1 || nb_y < 1 || nb_z < 1 || nb_x > (1u << 30) || nb_y > (1u << 20) || nb_z > (1u << 20)) {
-        set_error("hydro 3D step: interior sizes out of range (%zu x %zu x %zu)", nb_x, nb_y, nb_z);
+    const size_t nbv[3] = { nb_x, nb_y, nb_z };
+    if (nb_x < 1 || nb_y < 1 || nb_z < 1 || nb_x > (1u << 30) || nb_y > (1u << 20) || nb_z > (1u << 20) ||
+        !grid_fits(3, nbv)) {
+        set_error("hydro 3D step: interior sizes out of range or above 2^40 cells (%zu x %zu x %zu)", nb_x, nb_y,
+                  nb_z);
         return HYDRO_ERR_ARG;
     }
     if ((bc_y != HYDRO_BC_PERIODIC && bc_y != HYDRO_BC_REFLECTIVE) ||
@@ -270,6 +272,18 @@ int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const 
         set_error("hydro 3D step: unknown boundary mode (%d, %d)", bc_y, bc_z);
         return HYDRO_ERR_ARG;
     }
+    return 0;
+}
+
+template <typename T, typename TF = T, typename M = IeeeMath>
+int launch_step3d(const T* rhoE, const T* uvw, T* rhoE_next, T* uvw_next, const Consts3<T>& c, size_t nb_x,
+                  size_t nb_y, size_t nb_z, T DtDx, T DtDy, T DtDz, int bc_y, int bc_z, StepReduce* red,
+                  StepControl* ctl, T* Dt_next, double* traj_t, T* traj_dt, hipStream_t s, hipEvent_t ev_a = nullptr,
+                  hipEvent_t ev_b = nullptr, int wraps = 0)
+{
+    if (const int rc =
+            check_step3d(rhoE, uvw, rhoE_next, uvw_next, ctl != nullptr, Dt_next, nb_x, nb_y, nb_z, bc_y, bc_z))
+        return rc;
     Geom3 g;
     g.nb_x = static_cast<int>(nb_x);
     g.nb_y = static_cast<int>(nb_y);
@@ -342,7 +356,8 @@ template <typename T>
 int launch_ghost3d(T* rhoE, T* uvw, size_t nb_x, size_t nb_y, size_t nb_z, int bc_x, int bc_y, int bc_z,
                    const StepControl* ctl, hipStream_t s)
 {
-    if (!rhoE || !uvw || nb_x < 1 || nb_y < 1 || nb_z < 1) {
+    const size_t nbv[3] = { nb_x, nb_y, nb_z };
+    if (!rhoE || !uvw || nb_x < 1 || nb_y < 1 || nb_z < 1 || !grid_fits(3, nbv)) {
         set_error("hydro ghost fill 3D: bad arguments");
         return HYDRO_ERR_ARG;
     }
@@ -396,6 +411,9 @@ double timed_launcher3d(const T* d_rhoE, const T* d_uvw, T* d_rhoE_next, T* d_uv
                         T DtDz, void* stream)
 {
     clear_error();
+    if (const int rc = check_step3d(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, false, Dt_next, NB_X, NB_Y, NB_Z,
+                                    HYDRO_BC_REFLECTIVE, HYDRO_BC_REFLECTIVE))
+        return rc;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StepReduce* red = plain_reduce_state(s);
     EventPair3* ev = timing_events3();
@@ -484,6 +502,9 @@ extern "C" int hydro_step3d(const double* d_rhoE, const double* d_uvw, double* d
                             int bc_y, int bc_z, void* stream)
 {
     clear_error();
+    if (const int rc = check_step3d(d_rhoE, d_uvw, d_rhoE_next, d_uvw_next, false, Dt_next, NB_X, NB_Y, NB_Z, bc_y,
+                                    bc_z))
+        return rc;
     StepReduce* red = plain_reduce_state(static_cast<hipStream_t>(stream));
     if (!red) return HYDRO_ERR_HIP;
     const Consts3<double> c{ C_min, gamma, gamma_minus_one, divgamma, K };

@@ -567,6 +567,10 @@ int add_common(hydro_batch* b, const size_t* nb, const double* D, double C_min, 
             set_error("%s: nb[%d] = %zu out of [1, 2^30]", fn, d, nb[d]);
             return HYDRO_ERR_ARG;
         }
+    if (!grid_fits(b->dim, nb)) {
+        set_error("%s: grid of more than 2^40 cells", fn);
+        return HYDRO_ERR_ARG;
+    }
     size_t slot = 0;
     while (slot < b->cap && b->used[slot]) ++slot;
     if (slot == b->cap) {

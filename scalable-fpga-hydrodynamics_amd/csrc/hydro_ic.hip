@@ -149,6 +149,10 @@ int ic_impl(T* d_rhoE, T* d_mom, int dim, const size_t* nb, const size_t* base, 
             set_error("hydro_ic: sizes must be >= 1");
             return HYDRO_ERR_ARG;
         }
+    if (!grid_fits(dim, nb) || !grid_fits(dim, NB)) {
+        set_error("hydro_ic: grid of more than 2^40 cells");
+        return HYDRO_ERR_ARG;
+    }
     hipStream_t s = static_cast<hipStream_t>(stream);
     const IcSetup<T> q = ic_setup<T>(dim, NB, rho_in, rho_out, p_in, p_out, ic_mode, radius_frac, C_min, gamma);
 

@@ -107,6 +107,10 @@ bool run_arguments_ok(const char* what, int dim, const size_t* nb, const double*
             return false;
         }
     }
+    if (!grid_fits(dim, nb)) {
+        set_error("%s: grid of more than 2^40 cells", what);
+        return false;
+    }
     return true;
 }
 
@@ -665,7 +669,9 @@ int total_component(const void* d_rhoE, size_t NB_X, size_t NB_Y, size_t NB_Z, i
 {
     clear_error();
     if (precision == HYDRO_FP32_FLUX) precision = HYDRO_FP64;  // FP64 state
-    if (!d_rhoE || !out || NB_X < 1 || NB_Y < 1 || (precision != 64 && precision != 32)) {
+    const size_t nbv[3] = { NB_X, NB_Y, NB_Z };
+    if (!d_rhoE || !out || NB_X < 1 || NB_Y < 1 || (precision != 64 && precision != 32) ||
+        !grid_fits(NB_Z ? 3 : 2, nbv)) {
         set_error("%s: bad arguments", fn);
         return HYDRO_ERR_ARG;
     }

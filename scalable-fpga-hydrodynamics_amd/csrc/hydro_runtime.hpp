@@ -19,6 +19,25 @@ void set_exception_error(const char* fn);
 // Current device index, or -1 with the error recorded.
 int current_device();
 
+// Largest grid, ghosts included, that an entry point accepts: 2^40 cells, more than any
+// device's memory holds, so that every cell index the kernels form (at most three values
+// per cell) stays far inside 64 bits whatever sizes a caller passes.
+constexpr unsigned long long kMaxGridCells = 1ull << 40;
+
+// Whether interior sizes nb[0..dim) give at most kMaxGridCells cells with one ghost layer
+// on every side (sizes below 1 are left to the caller's own checks).
+inline bool grid_fits(int dim, const size_t* nb)
+{
+    unsigned long long cells = 1;
+    for (int d = 0; d < dim; ++d) {
+        if (nb[d] > kMaxGridCells) return false;
+        const unsigned long long n = static_cast<unsigned long long>(nb[d]) + 2;
+        if (cells > kMaxGridCells / n) return false;
+        cells *= n;
+    }
+    return true;
+}
+
 }  // namespace hydro
 
 #define HYDRO_HIP_OR(expr, retval)                                                                \

@@ -75,6 +75,33 @@ def test_bad_arguments_are_rejected_without_a_device(hydro):
     assert L.hydro_run_trajectory(None, 0, 1, None, None, None) == -1
 
 
+def test_grids_beyond_2_40_cells_are_rejected_before_any_device_work(hydro):
+    """Sizes whose grid (ghosts included) exceeds 2^40 cells, more than any device holds,
+    are refused before anything touches the device: the plain steps check their
+    arguments before binding the stream's reduction state.  The pointers are stand-ins
+    that no call may dereference."""
+    L = hydro.lib()
+    fake = C.c_void_p(0x1000)
+    big3 = (1 << 30, 1 << 20, 1 << 20)  # each axis within its own bound, 2^70 cells
+    assert L.hydro_step3d(fake, fake, fake, fake, fake, 0.1, 1.4, 0.4, 2.5, 1.1, *big3, 1e-3, 1e-3, 1e-3, 1, 1,
+                          None) == -1
+    assert "2^40" in hydro.last_error()
+    assert L.hydro_step2d(fake, fake, fake, fake, fake, 0.1, 1.4, 0.4, 2.5, 1.1, 1 << 30, 1 << 30, 1e-3, 1e-3, 1,
+                          None) == -1
+    assert "2^40" in hydro.last_error()
+    assert L.hydro_step2d(fake, fake, fake, fake, fake, 0.1, 1.4, 0.4, 2.5, 1.1, 0, 10, 1e-3, 1e-3, 1, None) == -1
+    assert L.hydro_ghost_fill3d(fake, fake, *big3, 0, 1, 1, None) == -1
+    assert L.hydro_ghost_fill2d(fake, fake, 1 << 30, 1 << 30, 0, 1, None) == -1
+    nb = (C.c_size_t * 3)(*big3)
+    assert L.hydro_ic(fake, fake, 3, nb, None, nb, 1.0, 1.2, 10.0, 0.1, 1, 0.1, 0.05, 1.4, None, None) == -1
+    assert "2^40" in hydro.last_error()
+    D = (C.c_double * 3)(1e-9, 1e-6, 1e-6)
+    assert L.hydro_run_create(3, nb, D, 1e-9, 1e-3, 1.0, 10, 64, None) is None
+    assert "2^40" in hydro.last_error()
+    mass = C.c_double()
+    assert L.hydro_total_mass(fake, *big3, 64, C.byref(mass), None) == -1
+
+
 def test_tuning_knobs_round_trip(hydro):
     for key, value, bad in [("rows2d", 96, 3), ("occ2d", 3, 2), ("occ2d", 8, 9), ("rows3d", 7, 1)]:
         old = hydro.tune_get(key)
