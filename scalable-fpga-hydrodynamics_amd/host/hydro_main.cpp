@@ -283,20 +283,26 @@ int run(const Options& o, const char* argv0)
     die_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
     size_t free_b = 0, total_b = 0;
     die_hip(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    size_t cells_g = 1;
-    for (int d = 0; d < kDim; ++d) cells_g *= nb[d] + 2;
-    const size_t n_re = 2 * cells_g, n_mo = static_cast<size_t>(kMom) * cells_g;
+    // The footprint in floating point first (hydro2d/main.cxx:546-549): sizes that wrap a
+    // size_t product (a negative -x read as size_t, as the reference reads it) then fail
+    // the memory check instead of passing it with a wrapped, small product.
+    double cells_d = 1.0;
+    for (int d = 0; d < kDim; ++d) cells_d *= static_cast<double>(nb[d]) + 2.0;
+    const double bytes_d = 2.0 * (2.0 + kMom) * cells_d * static_cast<double>(sizeof(T));
     const double one_gb = static_cast<double>(1ull << 30);
-    const size_t bytes_needed = 2 * (n_re + n_mo) * sizeof(T);  // hydro2d/main.cxx:546-549
     fprintf(stderr, "\n[rank 0] Device: %s (%s)\n", prop.name, prop.gcnArchName);
     fprintf(stderr, "[rank 0] Compute units              : %d\n", prop.multiProcessorCount);
     fprintf(stderr, "[rank 0] Max Global Memory          : %.2f GB\n", static_cast<double>(total_b) / one_gb);
-    fprintf(stderr, "[rank 0] Estimated Required Mem     : %.2f GB\n\n", static_cast<double>(bytes_needed) / one_gb);
-    if (bytes_needed > free_b) {
+    fprintf(stderr, "[rank 0] Estimated Required Mem     : %.2f GB\n\n", bytes_d / one_gb);
+    if (bytes_d > static_cast<double>(free_b)) {
         fprintf(stderr, "[rank 0] Insufficient device memory. Needed: %.2f GB, Available: %.2f GB\n",
-                static_cast<double>(bytes_needed) / one_gb, static_cast<double>(free_b) / one_gb);
+                bytes_d / one_gb, static_cast<double>(free_b) / one_gb);
         return 1;
     }
+    size_t cells_g = 1;  // fits: the footprint is below the device's free memory
+    for (int d = 0; d < kDim; ++d) cells_g *= nb[d] + 2;
+    const size_t n_re = 2 * cells_g, n_mo = static_cast<size_t>(kMom) * cells_g;
+    const size_t bytes_needed = 2 * (n_re + n_mo) * sizeof(T);
 
     // Constants, hydro2d/main.cxx:595-607 (3D: C = 0.4, hydro3d/main.cxx:649).
     const T rho_in = T(1.0), rho_out = T(1.2), p_in = T(10.0), p_out = T(0.1);

@@ -115,3 +115,14 @@ def test_driver_help_and_bad_option(gpu, tmp_path):
     assert p.returncode == 0 and "Hydrodynamics 2D" in p.stdout and "--bcx" in p.stdout
     p = subprocess.run([str(exe), "--bcy", "sideways"], capture_output=True, text=True, timeout=60)
     assert p.returncode != 0 and "Unknown boundary mode" in p.stderr
+
+
+def test_driver_refuses_sizes_beyond_the_device(gpu, tmp_path):
+    """Sizes are read as the reference reads them (size_t(stoi(...))), so -x -5 becomes a
+    huge size; the drivers' memory check, done in floating point, refuses it before
+    allocating anything (a size_t product would wrap to a small footprint)."""
+    for exe, args in (("hydro", ["-x", "-5", "-y", "10"]), ("hydro3d", ["-x", "10", "-y", "-3", "-z", "10"]),
+                      ("hydro3d", ["-x", "2000000", "-y", "2000000", "-z", "2000000"])):
+        p = subprocess.run([str(PKG / exe), *args, "-i", "1", "--no-output"], capture_output=True, text=True,
+                           timeout=60, cwd=tmp_path)
+        assert p.returncode != 0 and "Insufficient device memory" in p.stderr, (exe, args, p.stderr[-400:])
