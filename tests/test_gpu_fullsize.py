@@ -16,6 +16,8 @@ the oracle bit for bit.
 Both at full size also take their first steps bit for bit against the oracle run on the
 host (3 steps of 40000 x 20000, 5 of 512^3).
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -118,6 +120,48 @@ def _full_size_vs_oracle(gpu, oracle, nb, C, steps, ic_kw):
 def test_c2_full_size_first_steps_match_oracle(gpu, oracle):
     """40000 x 20000, the headline workload itself: its first 3 steps equal the oracle's."""
     _full_size_vs_oracle(gpu, oracle, (40000, 20000), 0.5, 3, {})
+
+
+@pytest.mark.skipif(os.environ.get("HYDRO_LONG") != "1",
+                    reason="about 4 minutes of oracle time on 16 host threads; run with HYDRO_LONG=1")
+def test_c2_full_size_100_steps_match_oracle(gpu, oracle):
+    """The headline configuration itself: 40000 x 20000, 100 steps, every interior value
+    and every Dt bit for bit against the oracle.  The oracle runs on the host in 10-step
+    chunks, each starting from the previous chunk's state and Dt (the loop state the
+    reference carries from step to step), with a progress line per chunk."""
+    import time
+
+    nb, steps, chunk = (40000, 20000), 100, 10
+    a, b = _state(nb)
+    k = gpu.constants(nb, 0.5)
+    dt0 = gpu.ic(a[0], a[1], k["C_min"])
+    run = gpu.Run(nb, k["D"], k["C_min"], dt0, 1e30, steps, gpu.FP64)
+    run.advance(a, b, steps)
+    st = run.status()
+    _, td = run.trajectory(steps)
+    run.close()
+    assert st["rc"] == 0 and st["it"] == steps, st
+    cur = a if st["current"] == 0 else b
+    got_r = cur[0].cpu().numpy()
+    got_m = cur[1].cpu().numpy()
+    del a, b, cur
+    torch.cuda.empty_cache()
+    print(f"[c2 {nb} x {steps}] GPU done, t={st['t']:.9e}", flush=True)
+    rhoE, mom, dt = oracle.ic2d(*nb)
+    assert dt == dt0
+    tds = []
+    t0 = time.perf_counter()
+    for c in range(steps // chunk):
+        res = oracle.run2d(rhoE, mom, dt, 1e30, chunk, nthreads=16)
+        tds.append(res["traj_dt"])
+        dt = res["dt"]
+        print(f"[c2 {nb} x {steps}] oracle {(c + 1) * chunk} steps after {time.perf_counter() - t0:.0f} s",
+              flush=True)
+    np.testing.assert_array_equal(np.concatenate(tds).view(np.uint64), td.view(np.uint64))
+    assert dt == st["dt"]
+    sl = (slice(1, -1),) * 2
+    np.testing.assert_array_equal(got_r[sl].view(np.uint64), rhoE[sl].view(np.uint64))
+    np.testing.assert_array_equal(got_m[sl].view(np.uint64), mom[sl].view(np.uint64))
 
 
 def test_c3_full_size_first_steps_match_oracle(gpu, oracle):
