@@ -164,6 +164,49 @@ def test_c2_full_size_100_steps_match_oracle(gpu, oracle):
     np.testing.assert_array_equal(got_m[sl].view(np.uint64), mom[sl].view(np.uint64))
 
 
+@pytest.mark.skipif(os.environ.get("HYDRO_LONG") != "1",
+                    reason="about 4 minutes of oracle time on 16 host threads; run with HYDRO_LONG=1")
+def test_c3_full_size_200_steps_match_oracle(gpu, oracle):
+    """The 3D configuration itself: 512^3 with the parameterised IC, all 200 steps in FP64,
+    every interior value and every Dt bit for bit against the oracle (20-step chunks on
+    the host, as in the 2D test), so the FP32-flux errors the bench reports against the
+    FP64 solve are its errors against the oracle."""
+    import time
+
+    nb, steps, chunk = (512, 512, 512), 200, 20
+    ic_kw = {"ic_mode": 1, "radius_frac": 0.1}
+    a, b = _state(nb)
+    k = gpu.constants(nb, 0.4)
+    dt0 = gpu.ic(a[0], a[1], k["C_min"], p_out=0.1, p_in=10.0, **ic_kw)
+    run = gpu.Run(nb, k["D"], k["C_min"], dt0, 1e30, steps, gpu.FP64)
+    run.advance(a, b, steps)
+    st = run.status()
+    _, td = run.trajectory(steps)
+    run.close()
+    assert st["rc"] == 0 and st["it"] == steps, st
+    cur = a if st["current"] == 0 else b
+    got_r = cur[0].cpu().numpy()
+    got_m = cur[1].cpu().numpy()
+    del a, b, cur
+    torch.cuda.empty_cache()
+    print(f"[c3 {nb} x {steps}] GPU done, t={st['t']:.9e}", flush=True)
+    rhoE, mom, dt = oracle.ic3d(*nb, **ic_kw)
+    assert dt == dt0
+    tds = []
+    t0 = time.perf_counter()
+    for c in range(steps // chunk):
+        res = oracle.run3d(rhoE, mom, dt, 1e30, chunk, nthreads=16)
+        tds.append(res["traj_dt"])
+        dt = res["dt"]
+        print(f"[c3 {nb} x {steps}] oracle {(c + 1) * chunk} steps after {time.perf_counter() - t0:.0f} s",
+              flush=True)
+    np.testing.assert_array_equal(np.concatenate(tds).view(np.uint64), td.view(np.uint64))
+    assert dt == st["dt"]
+    sl = (slice(1, -1),) * 3
+    np.testing.assert_array_equal(got_r[sl].view(np.uint64), rhoE[sl].view(np.uint64))
+    np.testing.assert_array_equal(got_m[sl].view(np.uint64), mom[sl].view(np.uint64))
+
+
 def test_c3_full_size_first_steps_match_oracle(gpu, oracle):
     """512^3 with the parameterised IC: its first 5 steps equal the oracle's."""
     _full_size_vs_oracle(gpu, oracle, (512, 512, 512), 0.4, 5, {"ic_mode": 1, "radius_frac": 0.1})
