@@ -167,3 +167,47 @@ def test_resident_run_f32_full_size_first_steps(gpu, oracle, dim, nb, steps):
     inner = interior2 if dim == 2 else interior3
     assert_bitwise(inner(got[0]), inner(rhoE), "rhoE")
     assert_bitwise(inner(got[1]), inner(mom), "momentum")
+
+
+@pytest.mark.skipif(__import__("os").environ.get("HYDRO_LONG") != "1",
+                    reason="minutes of oracle time on 16 host threads; run with HYDRO_LONG=1")
+@pytest.mark.parametrize("dim,nb,steps,chunk", [(2, (40000, 20000), 100, 10), (3, (512, 512, 512), 200, 20)])
+def test_resident_run_f32_full_size_full_length(gpu, oracle, dim, nb, steps, chunk):
+    """The FP32-state benchmark sizes for their configurations' full step counts: every
+    interior value and every Dt equal the oracle's FP32 loop, run on the host in chunks
+    that carry the state and Dt over (with a progress line per chunk)."""
+    import time
+
+    ic_kw = {} if dim == 2 else {"ic_mode": 1, "radius_frac": 0.1}
+    if dim == 2:
+        rhoE, mom, dt0 = oracle.ic2d(*nb, dtype=F32)
+        k = _k(gpu, nb, 0.5)
+    else:
+        rhoE, mom, dt0 = oracle.ic3d(*nb, dtype=F32, **ic_kw)
+        k = _k(gpu, nb, 0.4)
+    a = [to_dev(rhoE), to_dev(mom)]
+    b = [torch.zeros_like(a[0]), torch.zeros_like(a[1])]
+    run = gpu.Run(list(nb), k["D"], k["C_min"], dt0, 2.0, steps, 32)
+    run.advance(a, b, steps)
+    st = run.status()
+    assert st["rc"] == 0 and st["it"] == steps, st
+    _, td = run.trajectory(steps)
+    cur = a if st["current"] == 0 else b
+    got = (to_host(cur[0]), to_host(cur[1]))
+    run.close()
+    del a, b, cur
+    torch.cuda.empty_cache()
+    print(f"[fp32 full {nb} x {steps}] GPU done", flush=True)
+    dt, tds, t0 = dt0, [], time.perf_counter()
+    for c in range(steps // chunk):
+        step = oracle.run2d if dim == 2 else oracle.run3d
+        res = step(rhoE, mom, dt, 2.0, chunk, nthreads=16)
+        assert res["it"] == chunk
+        tds.append(np.asarray(res["traj_dt"], np.float64))
+        dt = res["dt"]
+        print(f"[fp32 full {nb} x {steps}] oracle {(c + 1) * chunk} steps after {time.perf_counter() - t0:.0f} s",
+              flush=True)
+    np.testing.assert_array_equal(np.concatenate(tds), np.asarray(td, np.float64))
+    inner = interior2 if dim == 2 else interior3
+    assert_bitwise(inner(got[0]), inner(rhoE), "rhoE")
+    assert_bitwise(inner(got[1]), inner(mom), "momentum")
